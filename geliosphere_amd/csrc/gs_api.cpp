@@ -1,0 +1,405 @@
+// gs_api.cpp - C ABI of the trajectory engine (host side).
+//
+// A gs_ctx owns one HIP stream, timing events and device buffers that are
+// reused across runs (the reference allocates and frees ~80 MB of managed
+// memory per run, CUDAKernel/src/OneDimensionBpGpuModel.cpp:16-42).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gs_api.h"
+#include "gs_kernels.h"
+
+using gs::DevOut;
+using gs::DevParams;
+
+namespace {
+
+const uint64_t kDefaultMaxSteps = 100000000ull;
+// 1.7 and 178.3 degrees as the reference writes them (GeliosphereCpuModel.cpp:112,217).
+const double kPi = 3.1415926535897932384626433832795;
+
+}  // namespace
+
+struct gs_ctx {
+    int device = 0;
+    int n_cu = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // device buffers, grown on demand
+    uint64_t *d_words = nullptr;   // queue | count | digits | stats for up to cap_sims sims
+    size_t cap_sims = 0;
+    gs_exit_record *d_recs = nullptr;
+    uint64_t cap_recs = 0;
+    uint64_t *d_nrec = nullptr;
+    DevParams *d_params = nullptr;
+    size_t cap_params = 0;
+    uint32_t block_override[4] = {0, 0, 0, 0};
+    uint32_t bpcu_override[4] = {0, 0, 0, 0};
+    gs_run_stats stats{};
+    std::string err;
+};
+
+namespace {
+
+gs_status fail(gs_ctx *ctx, gs_status st, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    return st;
+}
+
+#define HIP_TRY(ctx, call)                                                                  \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail((ctx), e_ == hipErrorOutOfMemory ? GS_ERR_OUT_OF_MEMORY : GS_ERR_HIP, \
+                        "%s failed: %s", #call, hipGetErrorString(e_));                     \
+    } while (0)
+
+size_t words_per_sim() { return 1 + GS_MAX_BINS + gs::kDigitsPerSim + gs::ST_COUNT_; }
+
+bool finite(double x) { return std::isfinite(x); }
+
+// Validation of a parameter block; msg receives the reason.
+bool check_params(const gs_params *p, std::string &msg) {
+    if (!p) { msg = "params is NULL"; return false; }
+    if (p->model < GS_MODEL_1D_FP || p->model > GS_MODEL_2D_GELIOSPHERE) { msg = "unknown model"; return false; }
+    if (p->injection < GS_INJ_LOG_UNIFORM || p->injection > GS_INJ_FIXED) { msg = "unknown injection"; return false; }
+    if (p->n_bins < 1 || p->n_bins > GS_MAX_BINS) { msg = "n_bins out of range"; return false; }
+    if (p->n_weights < 1 || p->n_weights > GS_MAX_WEIGHTS) { msg = "n_weights out of range"; return false; }
+    for (uint32_t i = 0; i < p->n_weights; ++i)
+        if (p->weights[i] < GS_W_POWERLAW || p->weights[i] > GS_W_BURGER) { msg = "unknown weight kind"; return false; }
+    if (!finite(p->K0) || p->K0 < 0) { msg = "K0 must be finite and >= 0"; return false; }
+    if (!finite(p->V) || !(p->V > 0)) { msg = "V must be finite and > 0"; return false; }
+    if (!finite(p->dt) || !(p->dt > 0)) { msg = "dt must be finite and > 0"; return false; }
+    if (!finite(p->r_inj) || !(p->r_inj > 0)) { msg = "r_inj must be finite and > 0"; return false; }
+    if (!finite(p->theta_inj) || !finite(p->tilt) || !finite(p->polarity) || !finite(p->ratio) ||
+        !finite(p->delta0)) { msg = "non-finite 2D parameter"; return false; }
+    if (!finite(p->T_min) || !finite(p->T_max) || !(p->T_min > 0)) { msg = "T_min must be finite and > 0"; return false; }
+    if (p->injection == GS_INJ_LOG_UNIFORM && !(p->T_max > p->T_min)) { msg = "T_max must exceed T_min"; return false; }
+    if (p->injection == GS_INJ_GRID && (p->grid_n == 0 || !(p->T_max >= p->T_min))) { msg = "bad grid"; return false; }
+    if (p->max_steps > 0xffffffffull) { msg = "max_steps above 2^32-1"; return false; }
+    if (p->n_particles > (1ull << 62) || p->first_particle > (1ull << 62)) { msg = "particle range too large"; return false; }
+    for (uint32_t i = 0; i <= p->n_bins; ++i)
+        if (!finite(p->bin_edges[i])) { msg = "non-finite bin edge"; return false; }
+    for (uint32_t i = 0; i < p->n_bins; ++i)
+        if (!(p->bin_edges[i] < p->bin_edges[i + 1])) { msg = "bin edges must ascend"; return false; }
+    return true;
+}
+
+DevParams derive(const gs_params &p) {
+    DevParams d;
+    std::memset(&d, 0, sizeof d);
+    d.model = p.model; d.injection = p.injection; d.flags = p.flags;
+    d.n_bins = p.n_bins; d.n_weights = p.n_weights; d.grid_n = p.grid_n;
+    for (int i = 0; i < GS_MAX_WEIGHTS; ++i) d.weights[i] = p.weights[i];
+    d.K0 = p.K0; d.V = p.V; d.dt = p.dt; d.r_inj = p.r_inj; d.theta_inj = p.theta_inj;
+    d.tilt = p.tilt; d.polarity = p.polarity; d.ratio = p.ratio; d.delta0 = p.delta0;
+    d.T_min = p.T_min; d.T_max = p.T_max;
+    d.log_ratio = (p.T_max > 0 && p.T_min > 0) ? std::log(p.T_max / p.T_min) : 0.0;
+    d.grid_step = p.grid_n ? (p.T_max - p.T_min) / p.grid_n : 0.0;
+    d.sqdt = std::sqrt(p.dt);
+    d.omega_v = 2.866e-6 / p.V;
+    d.omega2_v2 = 2.866e-6 * 2.866e-6 / (p.V * p.V);
+    d.th_lo = 1.7 * kPi / 180.;
+    d.th_hi = 178.3 * kPi / 180.0;
+    d.pol_theta = (p.flags & GS_FLAG_SOLARPROP_GPU_DRIFT) ? 1.0 : p.polarity;
+    d.n_particles = p.n_particles; d.first_particle = p.first_particle; d.seed = p.seed;
+    d.max_steps = p.max_steps ? p.max_steps : kDefaultMaxSteps;
+    for (uint32_t i = 0; i <= p.n_bins; ++i) d.edges[i] = p.bin_edges[i];
+    return d;
+}
+
+gs_status ensure_sims(gs_ctx *ctx, size_t sims) {
+    if (ctx->cap_sims >= sims) return GS_OK;
+    if (ctx->d_words) { (void)hipFree(ctx->d_words); ctx->d_words = nullptr; ctx->cap_sims = 0; }
+    HIP_TRY(ctx, hipMalloc((void **)&ctx->d_words, sims * words_per_sim() * sizeof(uint64_t)));
+    ctx->cap_sims = sims;
+    return GS_OK;
+}
+
+gs_status ensure_params(gs_ctx *ctx, size_t n) {
+    if (ctx->cap_params >= n) return GS_OK;
+    if (ctx->d_params) { (void)hipFree(ctx->d_params); ctx->d_params = nullptr; ctx->cap_params = 0; }
+    HIP_TRY(ctx, hipMalloc((void **)&ctx->d_params, n * sizeof(DevParams)));
+    ctx->cap_params = n;
+    return GS_OK;
+}
+
+gs_status ensure_recs(gs_ctx *ctx, uint64_t n) {
+    if (ctx->cap_recs >= n) return GS_OK;
+    if (ctx->d_recs) { (void)hipFree(ctx->d_recs); ctx->d_recs = nullptr; ctx->cap_recs = 0; }
+    HIP_TRY(ctx, hipMalloc((void **)&ctx->d_recs, n * sizeof(gs_exit_record)));
+    ctx->cap_recs = n;
+    return GS_OK;
+}
+
+// Layout of the word buffer for `sims` simulations.
+struct Layout {
+    uint64_t *queue, *count, *digits, *stats;
+};
+Layout layout(uint64_t *base, size_t sims) {
+    Layout L;
+    L.queue = base;
+    L.count = L.queue + sims;
+    L.digits = L.count + sims * GS_MAX_BINS;
+    L.stats = L.digits + sims * gs::kDigitsPerSim;
+    return L;
+}
+
+// Correctly rounded double of normalised 32-bit limbs.
+double limbs_to_double(const uint32_t *L) {
+    int h = GS_ACC_LIMBS - 1;
+    while (h >= 0 && L[h] == 0) --h;
+    if (h < 0) return 0.0;
+    unsigned __int128 top = 0;
+    int used = 0;
+    for (int i = h; i >= 0 && used < 4; --i, ++used) top = (top << 32) | L[i];
+    bool sticky = false;
+    for (int i = h - used; i >= 0; --i) sticky |= L[i] != 0;
+    int msb = 127;
+    while (!((top >> msb) & 1)) --msb;
+    int shift = msb - 52;
+    uint64_t mant;
+    if (shift > 0) {
+        const unsigned __int128 one = 1;
+        const unsigned __int128 rem = top & ((one << shift) - 1);
+        const unsigned __int128 half = one << (shift - 1);
+        mant = (uint64_t)(top >> shift);
+        if (rem > half || (rem == half && (sticky || (mant & 1)))) {
+            ++mant;
+            if (mant >> 53) { mant >>= 1; ++shift; }
+        }
+    } else {
+        mant = (uint64_t)(top << (-shift));
+    }
+    return std::ldexp((double)mant, shift + 32 * (h - used + 1) + GS_ACC_LSB_EXP);
+}
+
+// Convert one simulation's global words into a gs_spectrum.
+void fill_spectrum(const gs_params &p, const uint64_t *count, const uint64_t *digits,
+                   const uint64_t *stats, gs_spectrum *out) {
+    std::memset(out, 0, sizeof *out);
+    out->n_bins = p.n_bins;
+    out->n_weights = p.n_weights;
+    for (uint32_t b = 0; b < p.n_bins; ++b) out->count[b] = count[b];
+    uint64_t overflow = stats[gs::ST_OVERFLOW];
+    for (uint32_t k = 0; k < p.n_weights; ++k)
+        for (int s = 0; s < 2; ++s)
+            for (uint32_t b = 0; b < p.n_bins; ++b) {
+                const uint64_t *d = digits + ((size_t)(k * 2 + s) * GS_MAX_BINS + b) * GS_ACC_LIMBS;
+                uint32_t *L = out->acc[k][s][b];
+                uint64_t carry = 0;
+                for (int i = 0; i < GS_ACC_LIMBS; ++i) {
+                    const unsigned __int128 v = (unsigned __int128)d[i] + carry;
+                    L[i] = (uint32_t)v;
+                    carry = (uint64_t)(v >> 32);
+                }
+                if (carry) ++overflow;
+                const double v = limbs_to_double(L);
+                if (s == 0) out->sum_w[k][b] = v; else out->sum_w2[k][b] = v;
+            }
+    out->n_particles = stats[gs::ST_PARTICLES];
+    out->n_events = stats[gs::ST_EVENTS];
+    out->n_outside = stats[gs::ST_OUTSIDE];
+    out->total_steps = stats[gs::ST_STEPS];
+    out->n_nan = stats[gs::ST_NAN];
+    out->n_step_cap = stats[gs::ST_CAP];
+    out->n_acc_inexact = stats[gs::ST_INEXACT];
+    out->n_acc_overflow = overflow;
+}
+
+unsigned grid_for(gs_ctx *ctx, int model, int lds_bytes) {
+    int bpcu = (int)ctx->bpcu_override[model];
+    if (bpcu <= 0) {
+        if (gs_plain_occupancy(model, lds_bytes, &bpcu) != hipSuccess || bpcu <= 0) bpcu = 1;
+    }
+    return (unsigned)(bpcu * ctx->n_cu);
+}
+
+}  // namespace
+
+extern "C" {
+
+int gs_abi_version(void) { return GS_ABI_VERSION; }
+
+const char *gs_last_error(const gs_ctx *ctx) {
+    if (!ctx) return "no context";
+    return ctx->err.c_str();
+}
+
+gs_status gs_create(gs_ctx **out, int device) {
+    if (!out) return GS_ERR_INVALID_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return GS_ERR_NO_DEVICE;
+    if (device < 0 || device >= n) return GS_ERR_INVALID_ARG;
+    gs_ctx *ctx = new gs_ctx();
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    hipDeviceProp_t prop;
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+    if (e == hipSuccess) e = hipMalloc((void **)&ctx->d_nrec, sizeof(uint64_t));
+    if (e != hipSuccess) {
+        gs_destroy(ctx);
+        return GS_ERR_HIP;
+    }
+    ctx->n_cu = prop.multiProcessorCount;
+    *out = ctx;
+    return GS_OK;
+}
+
+void gs_destroy(gs_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->d_words) (void)hipFree(ctx->d_words);
+    if (ctx->d_recs) (void)hipFree(ctx->d_recs);
+    if (ctx->d_nrec) (void)hipFree(ctx->d_nrec);
+    if (ctx->d_params) (void)hipFree(ctx->d_params);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+gs_status gs_default_params(gs_params *p, int32_t model) {
+    if (!p || model < GS_MODEL_1D_FP || model > GS_MODEL_2D_GELIOSPHERE) return GS_ERR_INVALID_ARG;
+    std::memset(p, 0, sizeof *p);
+    const bool oneD = model == GS_MODEL_1D_FP || model == GS_MODEL_1D_BP;
+    p->model = model;
+    p->injection = GS_INJ_LOG_UNIFORM;
+    p->n_bins = 30;
+    p->n_weights = 1;
+    p->weights[0] = GS_W_POWERLAW;
+    p->grid_n = 10000;
+    p->K0 = 5e22 * 4.4683705e-27;
+    p->V = 400.0 * 6.68458712e-9;
+    p->dt = oneD ? 500.0 : 50.0;
+    p->r_inj = 1.0;
+    p->theta_inj = 90.0 * kPi / 180.0;
+    p->tilt = 10.0 * kPi / 180.0;
+    p->polarity = 1.0;
+    p->ratio = model == GS_MODEL_2D_SOLARPROP ? 0.02 : 0.2;
+    p->delta0 = 8.7e-5;
+    p->T_min = 0.01;
+    p->T_max = 100.0;
+    p->n_particles = 1000000;
+    p->seed = 1;
+    for (int k = 0; k <= 30; ++k) p->bin_edges[k] = 0.01 * std::pow(10.0, 4.0 * k / 30.0);
+    return GS_OK;
+}
+
+gs_status gs_validate_params(const gs_params *p, char *msg, uint32_t msg_len) {
+    std::string m;
+    const bool ok = check_params(p, m);
+    if (msg && msg_len) {
+        std::snprintf(msg, msg_len, "%s", ok ? "ok" : m.c_str());
+    }
+    return ok ? GS_OK : GS_ERR_INVALID_ARG;
+}
+
+gs_status gs_set_launch(gs_ctx *ctx, int32_t model, uint32_t block_threads, uint32_t blocks_per_cu) {
+    if (!ctx || model < 0 || model > 3) return GS_ERR_INVALID_ARG;
+    if (block_threads != 0 && block_threads != (uint32_t)gs::kBlockThreads)
+        return fail(ctx, GS_ERR_UNSUPPORTED, "block_threads must be 0 or %d", gs::kBlockThreads);
+    if (blocks_per_cu > 64) return fail(ctx, GS_ERR_INVALID_ARG, "blocks_per_cu above 64");
+    ctx->block_override[model] = block_threads;
+    ctx->bpcu_override[model] = blocks_per_cu;
+    return GS_OK;
+}
+
+gs_status gs_last_run_stats(const gs_ctx *ctx, gs_run_stats *st) {
+    if (!ctx || !st) return GS_ERR_INVALID_ARG;
+    *st = ctx->stats;
+    return GS_OK;
+}
+
+gs_status gs_run_plain(gs_ctx *ctx, const gs_params *p, gs_spectrum *out, gs_exit_record *recs,
+                       uint64_t rec_cap, uint64_t *n_recs) {
+    if (!ctx) return GS_ERR_INVALID_ARG;
+    if (!out) return fail(ctx, GS_ERR_INVALID_ARG, "out is NULL");
+    std::string msg;
+    if (!check_params(p, msg)) return fail(ctx, GS_ERR_INVALID_ARG, "invalid params: %s", msg.c_str());
+    const auto t_start = std::chrono::steady_clock::now();
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    gs_status st = ensure_sims(ctx, 1);
+    if (st != GS_OK) return st;
+    const bool want_recs = recs != nullptr && rec_cap > 0;
+    if (want_recs) {
+        st = ensure_recs(ctx, rec_cap);
+        if (st != GS_OK) return st;
+    }
+    st = ensure_params(ctx, 1);
+    if (st != GS_OK) return st;
+    const DevParams d = derive(*p);
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->d_params, &d, sizeof d, hipMemcpyHostToDevice, ctx->stream));
+    Layout L = layout(ctx->d_words, 1);
+    HIP_TRY(ctx, gs_launch_zero(ctx->d_words, words_per_sim(), ctx->stream));
+    HIP_TRY(ctx, gs_launch_zero(ctx->d_nrec, 1, ctx->stream));
+    DevOut O;
+    O.queue = L.queue;
+    O.g_count = L.count;
+    O.g_digits = L.digits;
+    O.g_stats = L.stats;
+    O.recs = want_recs ? ctx->d_recs : nullptr;
+    O.rec_cap = want_recs ? rec_cap : 0;
+    O.n_rec = ctx->d_nrec;
+    const int lds_bytes = gs::lds_words((int)p->n_bins, (int)p->n_weights) * (int)sizeof(uint64_t);
+    const unsigned grid = grid_for(ctx, p->model, lds_bytes);
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    HIP_TRY(ctx, gs_launch_plain(d, ctx->d_params, O, grid, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    std::vector<uint64_t> host(words_per_sim());
+    HIP_TRY(ctx, hipMemcpyAsync(host.data(), ctx->d_words, host.size() * sizeof(uint64_t),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    uint64_t nrec = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(&nrec, ctx->d_nrec, sizeof nrec, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    Layout H = layout(host.data(), 1);
+    fill_spectrum(*p, H.count, H.digits, H.stats, out);
+    if (want_recs) {
+        const uint64_t n = std::min(nrec, rec_cap);
+        HIP_TRY(ctx, hipMemcpy(recs, ctx->d_recs, n * sizeof(gs_exit_record), hipMemcpyDeviceToHost));
+        std::sort(recs, recs + n, [](const gs_exit_record &a, const gs_exit_record &b) {
+            return a.id != b.id ? a.id < b.id : a.sumac < b.sumac;
+        });
+    }
+    if (n_recs) *n_recs = H.stats[gs::ST_EVENTS];
+    ctx->stats.kernel_ms = ms;
+    ctx->stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    ctx->stats.lane_cycles = H.stats[gs::ST_CYCLES] * (uint64_t)gs::kWave * gs::kCycleSteps;
+    ctx->stats.active_cycles = H.stats[gs::ST_STEPS];
+    ctx->stats.grid_blocks = grid;
+    ctx->stats.block_threads = gs::kBlockThreads;
+    if (out->n_acc_overflow) return fail(ctx, GS_ERR_OVERFLOW, "weight accumulator overflow");
+    return GS_OK;
+}
+
+gs_status gs_run_batch(gs_ctx *ctx, const gs_params *sims, uint32_t n_sims, gs_spectrum *outs) {
+    if (!ctx) return GS_ERR_INVALID_ARG;
+    if (!sims || !outs) return fail(ctx, GS_ERR_INVALID_ARG, "NULL argument");
+    for (uint32_t i = 0; i < n_sims; ++i) {
+        gs_status st = gs_run_plain(ctx, &sims[i], &outs[i], nullptr, 0, nullptr);
+        if (st != GS_OK) return st;
+    }
+    return GS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,574 @@
+// gs_device.hpp - device-side pieces of the trajectory engine (gfx950).
+//
+// One lane integrates one particle at a time.  Per-particle state lives in
+// VGPRs; nothing is read from or written to HBM per step.  Events (exits,
+// 1 AU crossings) are folded into a workgroup-private LDS histogram whose
+// weighted sums are exact integers (fixed point, 32-bit digits in 64-bit
+// LDS words), so the result does not depend on the order in which lanes,
+// waves or workgroups fold their events.
+//
+// Model semantics follow the reference CPU path (thresholds, reflections,
+// order of normal draws); the arithmetic is re-associated for the GPU
+// (shared reciprocals, CSE of sin/cos, pow(Cb, k/2) as products of sqrt(Cb),
+// tan(acos x) = sqrt(1 - x^2) / x).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/gs_api.h"
+#include "gs_kernels.h"
+
+namespace gs {
+
+// Physical constants of the reference (CpuImplementations/include/Constants.hpp:63-127).
+constexpr double kM0 = 1.67261e-27;
+constexpr double kQ = 1.60219e-19;
+constexpr double kC = 2.99793e8;
+constexpr double kPi = 3.1415926535897932384626433832795;
+constexpr double kT0 = kM0 * kC * kC / (kQ * 1e9);  // proton rest energy [GeV]
+constexpr double kT0w = kM0 * kC * kC;              // [J]
+constexpr double kOmega = 2.866e-6;
+constexpr double kA = 3.4;
+constexpr double kKonvF = 9.0e-5 / 2.0;
+constexpr double kRh = 695510.0 / 150000000.0;
+// Float literals of the reference 1D B-p model, compared against doubles
+// (OneDimensionBpCpuModel.cpp:116,130,136).
+constexpr double kBetaMinBp = (double)0.01f;
+constexpr double kReflectBp = (double)0.3f;
+
+// ------------------------------------------------------------------ RNG
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+__device__ __forceinline__ void philox10(uint32_t &x0, uint32_t &x1, uint32_t &x2, uint32_t &x3,
+                                         uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int round = 0; round < 10; ++round) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * x0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * x2;
+        const uint32_t y0 = (uint32_t)(p1 >> 32) ^ x1 ^ k0;
+        const uint32_t y2 = (uint32_t)(p0 >> 32) ^ x3 ^ k1;
+        x0 = y0;
+        x1 = (uint32_t)p1;
+        x2 = y2;
+        x3 = (uint32_t)p0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// Per-lane normal stream.  Block k of particle id is
+// Philox(ctr = {id lo, id hi, k, 0}, key = {seed lo, seed hi}); its words
+// (x0,x1) and (x2,x3) give two Box-Muller pairs: u1 = (x+1) 2^-32 in (0,1],
+// u2 = x' 2^-32, z = sqrt(-2 ln u1) (cos, sin)(2 pi u2).
+struct NormalStream {
+    uint32_t x0, x1, x2, x3;
+    uint32_t blk;
+    int nidx;     // next normal within the block, 4 = exhausted
+    double zc;    // second member of the current pair
+
+    __device__ __forceinline__ void reset() {
+        blk = 0;
+        nidx = 4;
+    }
+    __device__ __forceinline__ double next(uint64_t pid, uint32_t k0, uint32_t k1) {
+        if (nidx == 4) {
+            x0 = (uint32_t)pid;
+            x1 = (uint32_t)(pid >> 32);
+            x2 = blk;
+            x3 = 0u;
+            philox10(x0, x1, x2, x3, k0, k1);
+            ++blk;
+            nidx = 0;
+        }
+        double z;
+        if ((nidx & 1) == 0) {
+            const uint32_t a = nidx == 0 ? x0 : x2;
+            const uint32_t b = nidx == 0 ? x1 : x3;
+            const double u1 = ((double)a + 1.0) * 0x1p-32;
+            const double u2 = (double)b * 0x1p-32;
+            const double rad = sqrt(-2.0 * log(u1));
+            double sn, cs;
+            sincospi(2.0 * u2, &sn, &cs);
+            z = rad * cs;
+            zc = rad * sn;
+        } else {
+            z = zc;
+        }
+        ++nidx;
+        return z;
+    }
+};
+
+__device__ __forceinline__ double injection_uniform(uint64_t seed, uint64_t pid) {
+    uint32_t x0 = (uint32_t)pid, x1 = (uint32_t)(pid >> 32), x2 = 0u, x3 = 1u;
+    philox10(x0, x1, x2, x3, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const uint64_t u53 = ((uint64_t)x0 << 21) | (uint64_t)(x1 >> 11);
+    return ((double)u53 + 0.5) * 0x1p-53;
+}
+
+// Injection energies: reference CpuImplementations/include/Constants.hpp:117-121,234-238.
+__constant__ double kSolarPropBins[30] = {
+    0.01, 0.015, 0.0225, 0.03375, 0.050625, 0.0759375, 0.113906, 0.170859, 0.256289, 0.384434,
+    0.57665, 0.864976, 1.29746, 1.9462, 2.91929, 4.37894, 6.56841, 9.85261, 14.7789, 22.1684,
+    33.2526, 49.8789, 74.8183, 112.227, 168.341, 252.512, 378.768, 568.151, 852.227, 1278.34};
+
+__device__ __forceinline__ double injection_energy(const DevParams &P, uint64_t pid) {
+    switch (P.injection) {
+    case GS_INJ_LOG_UNIFORM:
+        return P.T_min * exp(injection_uniform(P.seed, pid) * P.log_ratio);
+    case GS_INJ_GRID:
+        return P.T_min + ((double)(pid % P.grid_n) * P.grid_step);
+    case GS_INJ_SOLARPROP_BINS:
+        return kSolarPropBins[pid % 30];
+    default:
+        return P.T_min;
+    }
+}
+
+// ------------------------------------------------------------------ weights
+// Reference formulas: Algorithm/src/OneDimensionBpResults.cpp:51-60,
+// OneDimensionFpResults.cpp:55-64, TwoDimensionBpResults.cpp:45-78.
+__device__ __forceinline__ double p_si(double T) {
+    const double Tkinw = T * 1e9 * kQ;
+    const double Rig = sqrt(Tkinw * (Tkinw + (2 * kT0w))) / kQ;
+    return Rig * kQ / kC;
+}
+__device__ __forceinline__ double pgev2(double T) { return T * (T + 2.0 * kT0); }
+__device__ __forceinline__ double j_weber(double T) {
+    const double tem5 = 21.1 * exp(-2.8 * log(T));
+    const double tem6 = 1.0 + (5.85 * exp(-1.22 * log(T))) + (1.18 * exp(-2.54 * log(T)));
+    return tem5 / tem6;
+}
+__device__ __forceinline__ double j_usoskin(double T) {
+    const double Tr = 0.938;
+    const double tt = T + Tr;
+    const double t2 = tt + Tr;
+    const double beta = sqrt(T * t2) / tt;
+    double tem = (T + 0.67) / 1.67;
+    tem = exp(-3.93 * log(tem));
+    const double j = 2.7e3 * exp(1.12 * log(T)) * tem;
+    return j / (beta * beta);
+}
+__device__ __forceinline__ double j_burger(double T) {
+    double tem = sqrt((T * T) + (2.0 * T * 0.938));
+    const double j = 1.9 * 1.0e4 * exp(-2.78 * log(tem));
+    tem = 0.4866 * exp(-2.51 * log(tem));
+    return j / (1 + tem);
+}
+__device__ __forceinline__ double j_powerlaw(double T) { return exp(-2.78 * log(T + 0.94)); }
+__device__ __forceinline__ double f_yamada(double p) {
+    const double w = (kM0 * kM0 * kC * kC * kC * kC) + (p * p * kC * kC);
+    return exp(-1.85 * log(w)) / p;
+}
+
+struct Event {
+    double t_inj, t_exit, r, theta, sumac;
+};
+
+__device__ double event_weight(const DevParams &P, int kind, const Event &e) {
+    if (P.model == GS_MODEL_1D_FP) {
+        if (e.r < 0.3) return 0.0;
+        const double jac =
+            P.injection == GS_INJ_LOG_UNIFORM ? e.t_inj * P.log_ratio / (P.T_max - P.T_min) : 1.0;
+        const double es = exp(e.sumac);
+        if (kind == GS_W_YAMADA) {
+            const double p100 = p_si(e.t_inj), p1 = p_si(e.t_exit);
+            const double w0 = f_yamada(p100) / 1e45;
+            return w0 * p1 * p1 * es * jac;
+        }
+        if (kind == GS_W_POWERLAW) return j_powerlaw(e.t_inj) / pgev2(e.t_inj) * pgev2(e.t_exit) * es * jac;
+        const double pi = p_si(e.t_inj), po = p_si(e.t_exit);
+        const double j = kind == GS_W_WEBER ? j_weber(e.t_inj)
+                       : kind == GS_W_USOSKIN ? j_usoskin(e.t_inj) : j_burger(e.t_inj);
+        return j / (pi * pi) * (po * po) * es * jac;
+    }
+    if (kind == GS_W_YAMADA) {
+        const double p = p_si(e.t_exit), p1 = p_si(e.t_inj);
+        return f_yamada(p) * p1 * p1;
+    }
+    if (kind == GS_W_POWERLAW) return j_powerlaw(e.t_exit) * pgev2(e.t_inj) / pgev2(e.t_exit);
+    const double p = p_si(e.t_exit), p1 = p_si(e.t_inj);
+    const double j = kind == GS_W_WEBER ? j_weber(e.t_inj)
+                   : kind == GS_W_USOSKIN ? j_usoskin(e.t_exit) : j_burger(e.t_exit);
+    return (j / (p * p)) * p1 * p1;
+}
+
+// ------------------------------------------------------------------ histogram
+
+// Add v >= 0 exactly: v = m 2^e is cut into three 32-bit pieces that land in
+// consecutive digits.  Digits are 64-bit, so up to 2^32 pieces can be added
+// before any carry has to be propagated (done when the workgroup flushes).
+__device__ __forceinline__ void acc_add(uint64_t *dig, uint64_t *stat, double v) {
+    if (v == 0.0) return;
+    if (!(v > 0.0) || isinf(v)) {
+        atomicAdd((unsigned long long *)&stat[ST_OVERFLOW], 1ull);
+        return;
+    }
+    const uint64_t bits = (uint64_t)__double_as_longlong(v);
+    const int ex = (int)((bits >> 52) & 0x7ff);
+    uint64_t m = bits & ((1ull << 52) - 1);
+    int e2;
+    if (ex == 0) {
+        e2 = -1074;
+    } else {
+        m |= (1ull << 52);
+        e2 = ex - 1075;
+    }
+    int b = e2 - GS_ACC_LSB_EXP;
+    if (b < 0) {
+        if (-b >= 64) {
+            atomicAdd((unsigned long long *)&stat[ST_INEXACT], 1ull);
+            return;
+        }
+        const uint64_t lost = m & ((1ull << -b) - 1ull);
+        m >>= -b;
+        if (lost) atomicAdd((unsigned long long *)&stat[ST_INEXACT], 1ull);
+        b = 0;
+    }
+    const int limb = b >> 5, sh = b & 31;
+    const uint64_t v0 = m << sh;
+    const uint64_t hi = sh ? (m >> (64 - sh)) : 0ull;
+    const uint64_t d0 = v0 & 0xffffffffull, d1 = v0 >> 32, d2 = hi;
+    if (limb + (d2 ? 2 : d1 ? 1 : 0) >= GS_ACC_LIMBS) {
+        atomicAdd((unsigned long long *)&stat[ST_OVERFLOW], 1ull);
+        return;
+    }
+    if (d0) atomicAdd((unsigned long long *)&dig[limb], (unsigned long long)d0);
+    if (d1) atomicAdd((unsigned long long *)&dig[limb + 1], (unsigned long long)d1);
+    if (d2) atomicAdd((unsigned long long *)&dig[limb + 2], (unsigned long long)d2);
+}
+
+__device__ __forceinline__ int find_bin(const DevParams &P, double x) {
+    const int nb = (int)P.n_bins;
+    if (!(x >= P.edges[0]) || !(x < P.edges[nb])) return -1;
+    int lo = 0, hi = nb;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (x < P.edges[mid]) hi = mid; else lo = mid;
+    }
+    return lo;
+}
+
+// Fold one event into the workgroup histogram and, optionally, the record
+// buffer.
+__device__ void fold_event(const DevParams &P, uint64_t *lds, const Event &e, uint64_t pid,
+                           gs_exit_record *recs, uint64_t rec_cap, unsigned long long *n_rec) {
+    const int nb = (int)P.n_bins, nw = (int)P.n_weights;
+    uint64_t *cnt = lds;
+    uint64_t *dig = lds + nb;
+    uint64_t *stat = dig + nw * 2 * nb * GS_ACC_LIMBS;
+    atomicAdd((unsigned long long *)&stat[ST_EVENTS], 1ull);
+    if (recs) {
+        const unsigned long long i = atomicAdd(n_rec, 1ull);
+        if (i < rec_cap) {
+            gs_exit_record r;
+            r.t_inj = e.t_inj; r.t_exit = e.t_exit; r.r = e.r; r.theta = e.theta;
+            r.sumac = e.sumac; r.id = pid;
+            recs[i] = r;
+        }
+    }
+    const double x = P.model == GS_MODEL_1D_FP ? e.t_exit : e.t_inj;
+    const int b = find_bin(P, x);
+    if (b < 0) {
+        atomicAdd((unsigned long long *)&stat[ST_OUTSIDE], 1ull);
+        return;
+    }
+    atomicAdd((unsigned long long *)&cnt[b], 1ull);
+    for (int k = 0; k < nw; ++k) {
+        const double w = event_weight(P, P.weights[k], e);
+        acc_add(dig + ((k * 2 + 0) * nb + b) * GS_ACC_LIMBS, stat, w);
+        acc_add(dig + ((k * 2 + 1) * nb + b) * GS_ACC_LIMBS, stat, w * w);
+    }
+}
+
+// ------------------------------------------------------------------ models
+// Step outcome
+enum { STEP_CONTINUE = 0, STEP_EVENT = 1, STEP_EVENT_DONE = 2, STEP_DONE = 3 };
+
+// 1D B-p.  State: r [AU], P = pc/q [GV], beta (of the last computed T, which
+// a reflection does not restore, as in OneDimensionBpCpuModel.cpp:108-140).
+struct Bp {
+    double r, P, beta, t_inj;
+    __device__ __forceinline__ bool init(const DevParams &Pm, double T) {
+        t_inj = T;
+        P = sqrt(T * (T + 2.0 * kT0));
+        beta = P / (T + kT0);
+        r = Pm.r_inj;
+        return r < 100.0002;
+    }
+    __device__ __forceinline__ int step(const DevParams &Pm, NormalStream &ns, uint64_t pid,
+                                        uint32_t k0, uint32_t k1, Event &ev) {
+        const double Vb = -Pm.V;                       // Constants.hpp:213-218
+        const double K = Pm.K0 * beta * P;
+        const double inv_r = 1.0 / r;
+        const double Pn = P - (2.0 / 3.0) * Vb * Pm.dt * P * inv_r;
+        const double z = ns.next(pid, k0, k1);
+        const double dr = (Vb + 2.0 * K * inv_r) * Pm.dt + z * sqrt(2.0 * Pm.dt * K);
+        double rn = r + dr;
+        const double E = sqrt(kT0 * kT0 + Pn * Pn);
+        const double T = E - kT0;
+        const double bn = Pn / E;
+        int out = STEP_CONTINUE;
+        if (bn > kBetaMinBp && T < 200.0) {
+            if (rn > 100.0 && (rn - dr) < 100.0) {
+                ev.t_inj = t_inj; ev.t_exit = T; ev.r = rn; ev.theta = 0.0; ev.sumac = 0.0;
+                r = rn;
+                return STEP_EVENT_DONE;
+            }
+        } else if (bn < kBetaMinBp) {
+            out = STEP_DONE;
+        }
+        beta = bn;
+        if (rn < kReflectBp) {
+            rn = rn - dr;      // P keeps its pre-step value
+        } else {
+            P = Pn;
+        }
+        r = rn;
+        if (!(rn < 100.0002)) out = STEP_DONE;
+        return out;
+    }
+};
+
+// 1D F-p.  State: r, P [GV], beta and R (rigidity) of the last computed T,
+// sumac.  OneDimensionFpCpuModel.cpp:84-152.
+struct Fp {
+    double r, P, beta, R, sumac, t_inj;
+    __device__ __forceinline__ bool init(const DevParams &Pm, double T) {
+        t_inj = T;
+        P = sqrt(T * (T + 2.0 * kT0));
+        R = P;
+        beta = P / (T + kT0);
+        sumac = 0.0;
+        r = 100.0001;
+        return true;
+    }
+    __device__ __forceinline__ int step(const DevParams &Pm, NormalStream &ns, uint64_t pid,
+                                        uint32_t k0, uint32_t k1, Event &ev) {
+        const double K = Pm.K0 * beta * R;
+        const double inv_r = 1.0 / r;
+        const double z = ns.next(pid, k0, k1);
+        const double dr = (Pm.V + 2.0 * K * inv_r) * Pm.dt + z * sqrt(2.0 * Pm.dt * K);
+        const double Pn = P - (2.0 / 3.0) * Pm.V * Pm.dt * P * inv_r;
+        sumac += (4.0 / 3.0) * Pm.V * Pm.dt * inv_r;
+        double rn = r + dr;
+        const double E = sqrt(kT0 * kT0 + Pn * Pn);
+        const double T = E - kT0;
+        int out = STEP_CONTINUE;
+        if (beta > 0.01 && T < 100.0) {
+            // (r - 1) / ((r - dr) - 1) < 0, without the division
+            const double a = rn - 1.0, b = (rn - dr) - 1.0;
+            if ((a < 0.0 && b >= 0.0) || (a > 0.0 && b < 0.0)) {
+                ev.t_inj = t_inj; ev.t_exit = T; ev.r = rn; ev.theta = 0.0; ev.sumac = sumac;
+                out = STEP_EVENT;
+            }
+        }
+        const bool stop = beta < 0.01;
+        beta = Pn / E;
+        R = Pn;
+        if (rn < 0.1) {
+            rn = r;            // P keeps its pre-step value
+        } else {
+            P = Pn;
+        }
+        r = rn;
+        if (stop || !(rn < 100.0002)) out = (out == STEP_EVENT) ? STEP_EVENT_DONE : STEP_DONE;
+        return out;
+    }
+};
+
+// 2D SolarProp-like: SolarPropLikeCpuModel.cpp:93-256 (Kappl 2016, Eq. 13-18,
+// 26, 32-37, 42-44).
+struct Sp {
+    double r, theta, T, t_inj;
+    __device__ __forceinline__ bool init(const DevParams &Pm, double Tin) {
+        t_inj = Tin;
+        T = Tin;
+        r = Pm.r_inj;
+        theta = Pm.theta_inj;
+        return r < 100.0;
+    }
+    __device__ __forceinline__ int step(const DevParams &Pm, NormalStream &ns, uint64_t pid,
+                                        uint32_t k0, uint32_t k1, Event &ev) {
+        double s, c;
+        sincos(theta, &s, &c);
+        const double Rig = sqrt(T * (T + 2.0 * kT0));
+        const double inv_E = 1.0 / (T + kT0);
+        const double beta = Rig * inv_E;
+        const double r2 = r * r;
+        const double inv_r = 1.0 / r;
+        const double gamma = r * Pm.omega_v * s;
+        const double g2 = gamma * gamma;
+        const double opg = 1.0 + g2;
+        const double inv_opg = 1.0 / opg;
+        const double sq = sqrt(opg);
+        const double inv_sq = sq * inv_opg;
+        const double Rc = (Rig < 0.1) ? 0.1 : Rig;
+        const double Kpar = Pm.K0 * beta * Rc * (5.0 / 3.4) * r2 * inv_sq * (1.0 / 3.0);
+        const double Kper = Pm.ratio * Kpar;
+        const double Krr = Kper + (Kpar - Kper) * inv_opg;
+        const double Ktt = Kper;
+        const double kbr = Pm.K0 * beta * Rig;
+        const double dKrrTmp = 2.0 * r * Pm.omega2_v2 * s * s;
+        const double t1 = Pm.ratio * kbr * ((2.0 * r * sq) - (0.5 * r2 * dKrrTmp * inv_sq)) * inv_opg;
+        const double t2 = (1.0 - Pm.ratio) * kbr * ((2.0 * r * opg * sq) - (1.5 * r2 * dKrrTmp * sq)) *
+                          (inv_opg * inv_opg * inv_opg);
+        const double dKrr = (t1 + t2) * (5.0 / (3.0 * 3.4));
+        const double z1 = ns.next(pid, k0, k1);
+        double dr = ((-Pm.V) + (2.0 * Krr * inv_r) + dKrr) * Pm.dt + z1 * sqrt(2.0 * Pm.dt * Krr);
+        const double inv_s = 1.0 / s;
+        const double z2 = ns.next(pid, k0, k1);
+        double dth = Ktt * c * inv_s * inv_r * inv_r * Pm.dt * inv_opg + z2 * sqrt(2.0 * Pm.dt * Ktt) * inv_r;
+        const double dTkin = -(2.0 / 3.0) * Pm.V * Pm.dt * ((T + 2.0 * kT0) * inv_E) * T * inv_r;
+        // drifts
+        const double Larmor = (0.0225 / kA) * Rig * r2 * inv_sq;
+        const double sy = sin(Pm.tilt + Larmor * (kPi / 90.0) * inv_r);
+        const double x = sy / (kPi - sy);
+        const double alphaH = acos(x);
+        const double tanH = sqrt(1.0 - x * x) / x;
+        const double arg = (1.0 - theta * (2.0 / kPi)) * tanH;
+        const double inv_aH = 1.0 / alphaH;
+        const double f = atan(arg) * inv_aH;
+        const double dc = kKonvF * (2.0 / (3.0 * kA)) * Rig * beta * r * gamma * f * inv_opg * inv_opg;
+        const double DriftR = Pm.polarity * dc * c * inv_s;
+        const double DriftTheta = -Pm.pol_theta * dc * (2.0 + g2);
+        const double fprime = -(2.0 / kPi) * tanH * inv_aH / (1.0 + arg * arg);
+        const double DriftSheetR = Pm.polarity * kKonvF * (1.0 / (3.0 * kA)) * Rig * beta * r * gamma * fprime * inv_opg;
+        dr = dr + (DriftR + DriftSheetR) * Pm.dt;
+        dth = dth + DriftTheta * Pm.dt * inv_r;
+        r = r + dr;
+        theta = theta + dth;
+        if (theta < 0.0) theta = fabs(theta);
+        if (theta > 2.0 * kPi)
+            theta = theta - (2.0 * kPi);
+        else if (theta > kPi)
+            theta = (2.0 * kPi) - theta;
+        T = T - dTkin;
+        if (r < 0.0) r = -r;
+        if (r > 100.0) {
+            ev.t_inj = t_inj; ev.t_exit = T; ev.r = r; ev.theta = theta; ev.sumac = 0.0;
+            return STEP_EVENT_DONE;
+        }
+        return (r < 100.0) ? STEP_CONTINUE : STEP_DONE;
+    }
+};
+
+// 2D Geliosphere: GeliosphereCpuModel.cpp:95-407 (model notebook Eq. 16-44).
+// The B13 and B23 noise terms of the reference multiply exact zeros and are
+// not drawn; dr takes normals 1 and 2, dtheta normal 3.
+struct Ge {
+    double r, theta, T, t_inj;
+    __device__ __forceinline__ bool init(const DevParams &Pm, double Tin) {
+        t_inj = Tin;
+        T = Tin;
+        r = Pm.r_inj;
+        theta = Pm.theta_inj;
+        return r < 100.0;
+    }
+    __device__ __forceinline__ int step(const DevParams &Pm, NormalStream &ns, uint64_t pid,
+                                        uint32_t k0, uint32_t k1, Event &ev) {
+        constexpr double kf = 1.0 / (3.0 * (5.0 / 3.4));   // as written in GeliosphereCpuModel.cpp:210-299
+        double s, c;
+        sincos(theta, &s, &c);
+        const double Rig = sqrt(T * (T + 2.0 * kT0));
+        const double inv_E = 1.0 / (T + kT0);
+        const double beta = Rig * inv_E;
+        const double r2 = r * r;
+        const double inv_r = 1.0 / r;
+        const double inv_s = 1.0 / s;
+        const bool polar = (theta < Pm.th_lo) || (theta > Pm.th_hi);
+        const bool band = (theta > Pm.th_lo) && (theta < Pm.th_hi);
+        const double delta = polar ? 0.003 : Pm.delta0 * inv_s;
+        const double deltarh = delta * (1.0 / kRh);
+        const double deltarh2 = deltarh * deltarh;
+        const double gamma = r * Pm.omega_v * s;
+        const double g2 = gamma * gamma;
+        const double Cb = 1.0 + g2 + r2 * deltarh2;
+        const double inv_Cb = 1.0 / Cb;
+        const double sqCb = sqrt(Cb);
+        const double inv_sqCb = sqCb * inv_Cb;
+        const double inv_Cb15 = inv_Cb * inv_sqCb;
+        const double inv_Cb25 = inv_Cb * inv_Cb15;
+        const double Rc = (Rig < 0.1) ? 0.1 : Rig;
+        const double Kpar = Pm.K0 * beta * Rc * (5.0 / 3.4) * r2 * inv_sqCb * (1.0 / 3.0);
+        const double Kper = Pm.ratio * Kpar;
+        const double dK = Kpar - Kper;
+        const double Krr = Kper + dK * inv_Cb;
+        const double Ktt = Kper + r2 * deltarh2 * dK * inv_Cb;
+        const double Krt = deltarh * dK * r * inv_Cb;
+        const double inv_Ktt = 1.0 / Ktt;
+        const double sq2Ktt = sqrt(2.0 * Ktt);
+        const double B11 = sqrt(2.0 * (Krr * Ktt - Krt * Krt) * inv_Ktt);
+        const double B12 = Krt * inv_Ktt * sq2Ktt;
+        const double B22 = sq2Ktt * inv_r;
+        const double kbr = Pm.K0 * beta * Rig;
+        const double COmega = 2.0 * r * (Pm.omega2_v2 * s * s + deltarh2);
+        const double dKper = Pm.ratio * kbr * ((2.0 * r * sqCb) - (0.5 * r2 * COmega * inv_sqCb)) * kf * inv_Cb;
+        const double dKrr = dKper + (1.0 - Pm.ratio) * kbr *
+                                        ((2.0 * r * Cb * sqCb) - (1.5 * r2 * COmega * sqCb)) * kf *
+                                        (inv_Cb * inv_Cb * inv_Cb);
+        const double r2w = Pm.omega2_v2 * r2;
+        double dKtt;
+        if (band) {
+            const double CKtt = s * c * r2w;
+            const double dKtt1 = -Pm.ratio * kbr * r2 * CKtt * kf * inv_Cb15;
+            const double dKtt2 = (1.0 - Pm.ratio) * kbr * r2 * r2 * deltarh2;
+            const double dKtt4 = 3.0 * CKtt * inv_Cb25;
+            dKtt = dKtt1 - dKtt2 * dKtt4;
+        } else {
+            const double inv_s2 = inv_s * inv_s;
+            const double inv_s3 = inv_s2 * inv_s;
+            const double CKtt = s * c * r2w - r2 * (Pm.delta0 * Pm.delta0 / (kRh * kRh)) * c * inv_s3;
+            const double dKtt1 = -Pm.ratio * kbr * r2 * CKtt * kf * inv_Cb15;
+            const double dKtt2 = (1.0 - Pm.ratio) * kbr * r2 * r2 * (Pm.delta0 * Pm.delta0 / (kRh * kRh));
+            const double dKtt3 = -2.0 * c * inv_s3 * kf * inv_Cb15;
+            const double dKtt4 = 3.0 * CKtt * inv_s2 * kf * inv_Cb25;
+            dKtt = dKtt1 + dKtt2 * (dKtt3 - dKtt4);
+        }
+        const double dKrtr = (1.0 - Pm.ratio) * kbr * deltarh * r2 * kf * inv_Cb25;
+        const double krtt0 = (1.0 - Pm.ratio) * kbr * r2 * r * (kf / kRh) * inv_Cb25;
+        const double dKrtt = band ? -krtt0 * delta * 3.0 * g2 * c * inv_s
+                                  : -krtt0 * Pm.delta0 * c * inv_s * inv_s *
+                                        (1.0 - (2.0 * r2 * deltarh2) + (4.0 * g2));
+        const double z1 = ns.next(pid, k0, k1);
+        const double z2 = ns.next(pid, k0, k1);
+        double dr = ((-Pm.V) + (2.0 * Krr * inv_r) + dKrr) * Pm.dt + dKrtt * Pm.dt * inv_r +
+                    Krt * c * Pm.dt * inv_r * inv_s + (z1 * B11 + z2 * B12) * Pm.sqdt;
+        const double z3 = ns.next(pid, k0, k1);
+        double dth = Ktt * c * inv_s * inv_r * inv_r * Pm.dt + dKtt * Pm.dt * inv_r * inv_r +
+                     dKrtr * Pm.dt + 2.0 * Krt * Pm.dt * inv_r + z3 * B22 * Pm.sqdt;
+        const double dTkin = -(2.0 / 3.0) * Pm.V * Pm.dt * ((T + 2.0 * kT0) * inv_E) * T * inv_r;
+        // drifts (Kappl 2016 Eq. 26, 32-37 with Cb in place of 1 + gamma^2)
+        const double Larmor = (0.0225 / kA) * Rig * r2 * inv_sqCb;
+        const double sy = sin(Pm.tilt + Larmor * (kPi / 90.0) * inv_r);
+        const double x = sy / (kPi - sy);
+        const double alphaH = acos(x);
+        const double tanH = sqrt(1.0 - x * x) / x;
+        const double arg = (1.0 - theta * (2.0 / kPi)) * tanH;
+        const double inv_aH = 1.0 / alphaH;
+        const double f = atan(arg) * inv_aH;
+        const double dc = kKonvF * (2.0 / (3.0 * kA)) * Rig * beta * r * gamma * f * inv_Cb * inv_Cb;
+        const double DriftR = Pm.polarity * dc * c * inv_s;
+        const double DriftTheta = -Pm.polarity * dc * (2.0 + g2);
+        const double fprime = -(2.0 / kPi) * tanH * inv_aH / (1.0 + arg * arg);
+        const double DriftSheetR = Pm.polarity * kKonvF * (1.0 / (3.0 * kA)) * Rig * beta * r * gamma * fprime * inv_Cb;
+        dr = dr + (DriftR + DriftSheetR) * Pm.dt;
+        dth = dth + DriftTheta * Pm.dt * inv_r;
+        r = r + dr;
+        theta = theta + dth;
+        T = T - dTkin;
+        if (theta < 0.0) theta = fabs(theta);
+        if (theta > 2.0 * kPi)
+            theta = theta - (2.0 * kPi);
+        else if (theta > kPi)
+            theta = (2.0 * kPi) - theta;
+        if (r < 0.0) r = -r;
+        if (r > 100.0) {
+            ev.t_inj = t_inj; ev.t_exit = T; ev.r = r; ev.theta = theta; ev.sumac = 0.0;
+            return STEP_EVENT_DONE;
+        }
+        return (r < 100.0) ? STEP_CONTINUE : STEP_DONE;
+    }
+};
+
+}  // namespace gs

@@ -1,0 +1,77 @@
+// gs_kernels.h - host-visible interface of the trajectory kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/gs_api.h"
+
+namespace gs {
+constexpr int kWave = 64;
+constexpr int kBlockThreads = 256;
+constexpr int kCycleSteps = 4;     // steps between refills (whole Philox blocks for every model)
+constexpr uint64_t kChunk = 64;    // particle ids taken from the global queue per grab
+
+// Indices of the statistics words (LDS and global).
+enum Stat {
+    ST_PARTICLES = 0,
+    ST_EVENTS,
+    ST_OUTSIDE,
+    ST_STEPS,
+    ST_NAN,
+    ST_CAP,
+    ST_INEXACT,
+    ST_OVERFLOW,
+    ST_CYCLES,    // wave cycles issued (x 64 x kCycleSteps = lane-step slots)
+    ST_NREC,      // events offered to the record buffer
+    ST_COUNT_
+};
+
+// Parameters of one simulation as the kernels read them (host-derived
+// constants included).  Passed by value (plain runs) or from an array in
+// global memory (batched runs).
+struct DevParams {
+    int32_t model, injection;
+    uint32_t flags, n_bins, n_weights, grid_n;
+    int32_t weights[GS_MAX_WEIGHTS];
+    double K0, V, dt, r_inj, theta_inj, tilt, polarity, ratio, delta0, T_min, T_max;
+    // derived
+    double log_ratio;     // ln(T_max / T_min)
+    double grid_step;     // (T_max - T_min) / grid_n
+    double sqdt;          // sqrt(dt)
+    double omega_v;       // omega / V
+    double omega2_v2;     // omega^2 / V^2
+    double th_lo, th_hi;  // 1.7 deg and 178.3 deg in rad, as the reference writes them
+    double pol_theta;     // polarity used in the 2D theta drift
+    uint64_t n_particles, first_particle, seed, max_steps;
+    double edges[GS_MAX_BINS + 1];
+};
+
+
+// Global outputs of a launch.  Per simulation: count[GS_MAX_BINS],
+// digits[GS_MAX_WEIGHTS][2][GS_MAX_BINS][GS_ACC_LIMBS] (32-bit values in
+// 64-bit words), stats[ST_COUNT_].
+struct DevOut {
+    uint64_t *queue;
+    uint64_t *g_count;
+    uint64_t *g_digits;
+    uint64_t *g_stats;
+    gs_exit_record *recs;
+    uint64_t rec_cap;
+    uint64_t *n_rec;
+};
+
+// LDS layout per workgroup: count[n_bins] | digits[n_weights][2][n_bins][16] | stat[ST_COUNT_]
+__host__ __device__ inline int lds_words(int nb, int nw) {
+    return nb + nw * 2 * nb * GS_ACC_LIMBS + ST_COUNT_;
+}
+
+constexpr size_t kDigitsPerSim = (size_t)GS_MAX_WEIGHTS * 2 * GS_MAX_BINS * GS_ACC_LIMBS;
+}  // namespace gs
+
+hipError_t gs_launch_zero(uint64_t *p, size_t n, hipStream_t s);
+hipError_t gs_plain_occupancy(int model, int lds_bytes, int *blocks_per_cu);
+// P: host copy (launch shape), dP: the same block in device memory.
+hipError_t gs_launch_plain(const gs::DevParams &P, const gs::DevParams *dP, const gs::DevOut &O,
+                           unsigned grid, hipStream_t s);

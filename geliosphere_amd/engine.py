@@ -1,0 +1,137 @@
+"""ctypes front end of libgeliosphere_amd.so (the gfx950 trajectory engine).
+
+The library is the product: this module only loads it and moves plain C
+structs across the C ABI declared in include/gs_api.h.  There is no CPU
+fallback: if the library or a HIP device is missing, the calls raise.
+"""
+import ctypes
+import os
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgeliosphere_amd.so")
+_lib = None
+
+EXPORTS = ("gs_abi_version", "gs_create", "gs_destroy", "gs_last_error", "gs_default_params",
+           "gs_validate_params", "gs_run_plain", "gs_run_batch", "gs_set_launch",
+           "gs_last_run_stats")
+
+
+class GsError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("gs status %d: %s" % (status, msg))
+        self.status = status
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libgeliosphere_amd.so is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        vp = ctypes.c_void_p
+        L.gs_abi_version.restype = ctypes.c_int
+        L.gs_create.argtypes = [P(vp), ctypes.c_int]
+        L.gs_create.restype = ctypes.c_int
+        L.gs_destroy.argtypes = [vp]
+        L.gs_destroy.restype = None
+        L.gs_last_error.argtypes = [vp]
+        L.gs_last_error.restype = ctypes.c_char_p
+        L.gs_default_params.argtypes = [P(abi.GsParams), ctypes.c_int32]
+        L.gs_default_params.restype = ctypes.c_int
+        L.gs_validate_params.argtypes = [P(abi.GsParams), ctypes.c_char_p, ctypes.c_uint32]
+        L.gs_validate_params.restype = ctypes.c_int
+        L.gs_run_plain.argtypes = [vp, P(abi.GsParams), P(abi.GsSpectrum), P(abi.GsExitRecord),
+                                   ctypes.c_uint64, P(ctypes.c_uint64)]
+        L.gs_run_plain.restype = ctypes.c_int
+        L.gs_run_batch.argtypes = [vp, P(abi.GsParams), ctypes.c_uint32, P(abi.GsSpectrum)]
+        L.gs_run_batch.restype = ctypes.c_int
+        L.gs_set_launch.argtypes = [vp, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32]
+        L.gs_set_launch.restype = ctypes.c_int
+        L.gs_last_run_stats.argtypes = [vp, P(abi.GsRunStats)]
+        L.gs_last_run_stats.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def validate(p):
+    buf = ctypes.create_string_buffer(256)
+    rc = lib().gs_validate_params(ctypes.byref(p), buf, 256)
+    return rc, buf.value.decode()
+
+
+def default_params(model):
+    p = abi.GsParams()
+    rc = lib().gs_default_params(ctypes.byref(p), model)
+    if rc:
+        raise GsError(rc, "gs_default_params")
+    return p
+
+
+class Engine:
+    """One gs_ctx on one HIP device."""
+
+    def __init__(self, device=0):
+        self._ctx = ctypes.c_void_p()
+        rc = lib().gs_create(ctypes.byref(self._ctx), device)
+        if rc:
+            raise GsError(rc, "gs_create(device=%d) failed" % device)
+
+    def close(self):
+        if self._ctx:
+            lib().gs_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc:
+            raise GsError(rc, "%s: %s" % (what, lib().gs_last_error(self._ctx).decode()))
+
+    def run_plain(self, p, records=False, rec_cap=None):
+        """Returns (spectrum, records or None, n_events)."""
+        s = abi.GsSpectrum()
+        n = ctypes.c_uint64(0)
+        if records:
+            cap = rec_cap if rec_cap is not None else max(16, int(2 * p.n_particles))
+            recs = (abi.GsExitRecord * cap)()
+            rc = lib().gs_run_plain(self._ctx, ctypes.byref(p), ctypes.byref(s), recs, cap, ctypes.byref(n))
+            self._check(rc, "gs_run_plain")
+            return s, recs[: min(cap, n.value)], n.value
+        rc = lib().gs_run_plain(self._ctx, ctypes.byref(p), ctypes.byref(s), None, 0, ctypes.byref(n))
+        self._check(rc, "gs_run_plain")
+        return s, None, n.value
+
+    def run_plain_status(self, p):
+        """Status code of a run (no raise), for error-path tests."""
+        s = abi.GsSpectrum()
+        return lib().gs_run_plain(self._ctx, ctypes.byref(p), ctypes.byref(s), None, 0, None)
+
+    def run_batch(self, sims):
+        n = len(sims)
+        arr = (abi.GsParams * n)(*sims)
+        outs = (abi.GsSpectrum * n)()
+        rc = lib().gs_run_batch(self._ctx, arr, n, outs)
+        self._check(rc, "gs_run_batch")
+        return list(outs)
+
+    def set_launch(self, model, block_threads=0, blocks_per_cu=0):
+        self._check(lib().gs_set_launch(self._ctx, model, block_threads, blocks_per_cu), "gs_set_launch")
+
+    def last_stats(self):
+        st = abi.GsRunStats()
+        self._check(lib().gs_last_run_stats(self._ctx, ctypes.byref(st)), "gs_last_run_stats")
+        return st

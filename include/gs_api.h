@@ -1,0 +1,212 @@
+/*
+ * gs_api.h - C ABI of the gfx950 SDE trajectory engine (heliospheric
+ * cosmic-ray modulation, four models: 1D F-p, 1D B-p, 2D SolarProp-like,
+ * 2D Geliosphere).
+ *
+ * Plain C types only: no HIP, torch or C++ types cross this boundary.
+ * Every call returns gs_status; GS_OK == 0.  A gs_ctx is bound to one device
+ * and is not re-entrant; distinct contexts may be used from distinct threads.
+ *
+ * Which reference interface each entry point replaces:
+ *
+ *   gs_run_plain   replaces XGpuModel::prepareAndRunSimulation(ParamsCarrier*)
+ *                  of the four GPU models (reference
+ *                  CUDAKernel/include/AbstractGpuSimulation.hpp:31;
+ *                  CUDAKernel/src/OneDimensionFpGpuModel.cpp:7-40,
+ *                  OneDimensionBpGpuModel.cpp:7-43,
+ *                  SolarPropLikeGpuModel.cpp:7-44,
+ *                  GeliosphereGpuModel.cpp:7-44) together with the
+ *                  run*Simulation launch loops they call
+ *                  (OneDimensionBpModel.cu:149-200 etc.) and the log.dat
+ *                  re-read + binning of the Results pass
+ *                  (Algorithm/src/OneDimensionBpResults.cpp:14-113,
+ *                  OneDimensionFpResults.cpp:13-114,
+ *                  TwoDimensionBpResults.cpp:13-134): the histogram is built
+ *                  on the device, exit records are optional.
+ *   gs_params      replaces the __constant__ parameter block filled by
+ *                  setConstants (CUDAKernel/src/CosmicConstants.cu:14-111)
+ *                  and the simulationInputX structs
+ *                  (CUDAKernel/include/OneDimensionBpModel.cuh:69-81).
+ *   gs_run_batch   replaces the sequential row loop of
+ *                  BatchRun::runAlgorithm (Algorithm/src/BatchRun.cpp:50-78).
+ *   gs_create /    replace the per-run cudaMalloc/cudaFree of the GPU model
+ *   gs_destroy     wrappers (OneDimensionBpGpuModel.cpp:16-42).
+ *   gs_last_error  replaces gpuAssert's print-and-exit
+ *                  (CUDAKernel/include/CudaErrorCheck.cuh:25-41).
+ */
+#ifndef GS_API_H
+#define GS_API_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GS_ABI_VERSION 1
+
+#define GS_MAX_BINS 64
+#define GS_MAX_WEIGHTS 4
+/* Exact weight accumulator: GS_ACC_LIMBS limbs of 32 bits; bit 0 of limb 0
+ * is worth 2^GS_ACC_LSB_EXP.  Values below 2^GS_ACC_LSB_EXP are truncated
+ * (counted in n_acc_inexact), totals must stay below 2^(GS_ACC_LSB_EXP+512). */
+#define GS_ACC_LIMBS 16
+#define GS_ACC_LSB_EXP (-320)
+
+typedef enum gs_status {
+    GS_OK = 0,
+    GS_ERR_INVALID_ARG = 1,
+    GS_ERR_HIP = 2,
+    GS_ERR_OUT_OF_MEMORY = 3,
+    GS_ERR_OVERFLOW = 4,
+    GS_ERR_NO_DEVICE = 5,
+    GS_ERR_IO = 6,
+    GS_ERR_UNSUPPORTED = 7
+} gs_status;
+
+typedef enum gs_model {
+    GS_MODEL_1D_FP = 0,          /* "1D Fp"  forward-in-time  (-F) */
+    GS_MODEL_1D_BP = 1,          /* "1D Bp"  backward-in-time (-B) */
+    GS_MODEL_2D_SOLARPROP = 2,   /* "2D SolarProp-like"       (-E) */
+    GS_MODEL_2D_GELIOSPHERE = 3  /* "2D Geliosphere"          (-T) */
+} gs_model;
+
+typedef enum gs_injection {
+    /* T = T_min (T_max/T_min)^u, u from the particle's Philox injection draw */
+    GS_INJ_LOG_UNIFORM = 0,
+    /* T = T_min + (id mod grid_n) (T_max - T_min) / grid_n
+     * (reference CpuImplementations/include/Constants.hpp:234-238) */
+    GS_INJ_GRID = 1,
+    /* T = SolarProp energy table[id mod 30]
+     * (reference CpuImplementations/include/Constants.hpp:117-121) */
+    GS_INJ_SOLARPROP_BINS = 2,
+    /* T = T_min for every particle */
+    GS_INJ_FIXED = 3
+} gs_injection;
+
+/* Local interstellar spectra used to weight an event.  J is a differential
+ * intensity; weight = J_LIS(T_lis) / p_lis^2 * p_obs^2 where (T_lis, p_lis)
+ * is the particle at the outer boundary and p_obs at the observer. */
+typedef enum gs_weight {
+    GS_W_POWERLAW = 0,  /* J = (T + 0.94)^-2.78                              */
+    GS_W_YAMADA = 1,    /* f = (m0^2c^4 + p^2c^2)^-1.85 / p  (SI)
+                           (Algorithm/src/OneDimensionBpResults.cpp:58-60)   */
+    GS_W_WEBER = 2,     /* Webber-Higbie, at T_inj (TwoDimensionBpResults.cpp:50-54) */
+    GS_W_USOSKIN = 3,   /* Usoskin 2005 (TwoDimensionBpResults.cpp:59-67)    */
+    GS_W_BURGER = 4     /* Burger 2000  (TwoDimensionBpResults.cpp:72-78)    */
+} gs_weight;
+
+/* gs_params.flags */
+#define GS_FLAG_SOLARPROP_GPU_DRIFT 0x1u /* 2D SolarProp-like: theta drift with
+                                            polarity fixed to +1, as the
+                                            reference GPU kernel does
+                                            (SolarPropLikeModel.cu:213,
+                                            CosmicConstants.cu:37) */
+
+typedef struct gs_params {
+    int32_t model;               /* gs_model */
+    int32_t injection;           /* gs_injection */
+    uint32_t flags;
+    uint32_t n_bins;             /* 1 .. GS_MAX_BINS */
+    uint32_t n_weights;          /* 1 .. GS_MAX_WEIGHTS */
+    int32_t weights[GS_MAX_WEIGHTS]; /* gs_weight per weight slot */
+    uint32_t grid_n;             /* GS_INJ_GRID only */
+    double K0;          /* diffusion coefficient [AU^2/s]                  */
+    double V;           /* solar wind speed [AU/s], > 0; 1D B-p uses -V      */
+    double dt;          /* time step [s]                                     */
+    double r_inj;       /* injection radius [AU] (backward models)           */
+    double theta_inj;   /* injection colatitude [rad] (2D models)            */
+    double tilt;        /* current-sheet tilt alpha_M [rad] (2D models)      */
+    double polarity;    /* +1 / -1 (2D models)                               */
+    double ratio;       /* K_perp / K_par (2D models)                        */
+    double delta0;      /* polar-field constant C_delta (2D Geliosphere)     */
+    double T_min;       /* injection energy range [GeV]                      */
+    double T_max;
+    uint64_t n_particles;    /* particles of this run                        */
+    uint64_t first_particle; /* id of the first particle (shards)            */
+    uint64_t seed;           /* Philox key                                   */
+    uint64_t max_steps;      /* per-particle step cap, 0 = library default   */
+    double bin_edges[GS_MAX_BINS + 1]; /* ascending; bin i = [e_i, e_{i+1}) */
+} gs_params;
+
+typedef struct gs_spectrum {
+    uint32_t n_bins;
+    uint32_t n_weights;
+    /* events per bin: exits (B-p, 2D) or 1 AU crossings (F-p) */
+    uint64_t count[GS_MAX_BINS];
+    /* correctly rounded values of the exact sums below */
+    double sum_w[GS_MAX_WEIGHTS][GS_MAX_BINS];
+    double sum_w2[GS_MAX_WEIGHTS][GS_MAX_BINS];
+    uint64_t n_particles;    /* particles integrated                         */
+    uint64_t n_events;       /* events recorded (all bins + outside)         */
+    uint64_t n_outside;      /* events whose bin variable is outside edges   */
+    uint64_t total_steps;    /* Euler-Maruyama steps over all particles      */
+    uint64_t n_nan;          /* particles stopped by a NaN state             */
+    uint64_t n_step_cap;     /* particles stopped by max_steps               */
+    uint64_t n_acc_inexact;  /* weights truncated below the accumulator LSB  */
+    uint64_t n_acc_overflow; /* accumulator overflowed (sums invalid)        */
+    /* exact sums, normalised 32-bit limbs, little-endian; [w][0]=sum w,
+     * [w][1]=sum w^2 */
+    uint32_t acc[GS_MAX_WEIGHTS][2][GS_MAX_BINS][GS_ACC_LIMBS];
+} gs_spectrum;
+
+/* One exit (B-p, 2D) or one 1 AU crossing (F-p). 48 bytes. */
+typedef struct gs_exit_record {
+    double t_inj;    /* injection kinetic energy [GeV]                       */
+    double t_exit;   /* kinetic energy at the event [GeV]                    */
+    double r;        /* radius at the event [AU]                             */
+    double theta;    /* colatitude at the event [rad] (2D), 0 otherwise      */
+    double sumac;    /* F-p adiabatic loss integral, 0 otherwise             */
+    uint64_t id;     /* particle id                                          */
+} gs_exit_record;
+
+typedef struct gs_run_stats {
+    double kernel_ms;        /* trajectory kernel time (HIP events)          */
+    double total_ms;         /* whole call                                   */
+    uint64_t lane_cycles;    /* lane-cycles issued (active + idle)           */
+    uint64_t active_cycles;  /* lane-cycles with a live particle             */
+    uint32_t grid_blocks;
+    uint32_t block_threads;
+} gs_run_stats;
+
+typedef struct gs_ctx gs_ctx;
+
+/* Open a context on HIP device `device`. */
+gs_status gs_create(gs_ctx **ctx, int device);
+void gs_destroy(gs_ctx *ctx);
+/* Message of the last failing call on ctx (never NULL). */
+const char *gs_last_error(const gs_ctx *ctx);
+int gs_abi_version(void);
+
+/* Fill *p with the defaults of `model` (K0 = 5e22 cm^2/s, V = 400 km/s,
+ * dt = 500 s (1D) / 50 s (2D), r = 1 AU, theta = 90 deg, log-uniform
+ * 0.01-100 GeV injection, 30 log bins, power-law LIS). Host only. */
+gs_status gs_default_params(gs_params *p, int32_t model);
+/* Validate parameters without running. Host only. */
+gs_status gs_validate_params(const gs_params *p, char *msg, uint32_t msg_len);
+
+/* Run one simulation: particles [first_particle, first_particle+n_particles).
+ * out: histogram (required).  recs/rec_cap/n_recs: optional exit records
+ * (recs may be NULL); *n_recs receives the number of events, which can
+ * exceed rec_cap (only rec_cap are stored, sorted by particle id, then by
+ * event order within a particle). */
+gs_status gs_run_plain(gs_ctx *ctx, const gs_params *p, gs_spectrum *out,
+                       gs_exit_record *recs, uint64_t rec_cap,
+                       uint64_t *n_recs);
+
+/* Run n_sims independent simulations in one persistent launch.  outs[i] is
+ * identical, bit for bit, to gs_run_plain(sims[i]). */
+gs_status gs_run_batch(gs_ctx *ctx, const gs_params *sims, uint32_t n_sims,
+                       gs_spectrum *outs);
+
+/* Override the launch shape used for `model` (0 = automatic). */
+gs_status gs_set_launch(gs_ctx *ctx, int32_t model, uint32_t block_threads,
+                        uint32_t blocks_per_cu);
+/* Statistics of the last run on ctx. */
+gs_status gs_last_run_stats(const gs_ctx *ctx, gs_run_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GS_API_H */

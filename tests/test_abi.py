@@ -1,0 +1,74 @@
+"""The C-ABI library: it loads, exports every entry point include/gs_api.h
+declares, and its host-side logic (validation, defaults, errors without a
+device) behaves.  CPU only: no kernel is launched."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from geliosphere_amd import abi, engine
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "gs_api.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gs_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = engine.lib()
+    names = declared_functions()
+    assert set(names) == set(engine.EXPORTS)
+    for n in names:
+        assert hasattr(L, n), n
+
+
+def test_abi_version_and_struct_sizes():
+    assert engine.lib().gs_abi_version() == 1
+    assert ctypes.sizeof(abi.GsExitRecord) == 48
+    # gs_params: 5 x 4 B + 4 x 4 B + 4 B + 11 x 8 B + 4 x 8 B + 65 x 8 B, no padding
+    assert ctypes.sizeof(abi.GsParams) == 4 * 5 + 4 * 4 + 4 + 11 * 8 + 4 * 8 + 65 * 8
+
+
+def test_default_params_are_the_baseline_configuration():
+    p = engine.default_params(abi.MODEL_1D_BP)
+    assert p.K0 == pytest.approx(5e22 * 4.4683705e-27)
+    assert p.V == pytest.approx(400 * 6.68458712e-9)
+    assert p.dt == 500.0 and p.r_inj == 1.0
+    assert p.n_bins == 30 and p.injection == abi.INJ_LOG_UNIFORM
+    for k in range(31):
+        assert p.bin_edges[k] == pytest.approx(0.01 * 10 ** (4 * k / 30), rel=1e-14)
+    q = engine.default_params(abi.MODEL_2D_GELIOSPHERE)
+    assert q.dt == 50.0 and q.ratio == 0.2
+    assert engine.default_params(abi.MODEL_2D_SOLARPROP).ratio == 0.02
+
+
+@pytest.mark.parametrize("field,value", [
+    ("dt", 0.0), ("dt", float("nan")), ("V", -1.0), ("K0", float("inf")), ("n_bins", 0),
+    ("n_bins", 65), ("n_weights", 0), ("model", 7), ("injection", 9), ("max_steps", 1 << 33),
+    ("T_min", 0.0), ("r_inj", float("nan")),
+])
+def test_validation_rejects(field, value):
+    p = abi.make_params(abi.MODEL_1D_BP)
+    setattr(p, field, value)
+    rc, msg = engine.validate(p)
+    assert rc == abi.GS_ERR_INVALID_ARG and msg != "ok"
+
+
+def test_validation_rejects_unsorted_edges():
+    p = abi.make_params(abi.MODEL_1D_BP)
+    p.bin_edges[5] = p.bin_edges[4]
+    assert engine.validate(p)[0] == abi.GS_ERR_INVALID_ARG
+    assert engine.validate(abi.make_params(abi.MODEL_1D_BP))[0] == abi.GS_OK
+
+
+def test_create_without_device_fails_cleanly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    ctx = ctypes.c_void_p()
+    assert engine.lib().gs_create(ctypes.byref(ctx), 0) == abi.GS_ERR_NO_DEVICE
+    assert not ctx.value
