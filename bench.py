@@ -34,7 +34,23 @@ def parse():
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--model", choices=["bp+fp", "bp", "fp", "sp", "ge"], default="bp+fp",
+                    help="models run per step (the headline workload is bp+fp)")
+    ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
     return ap.parse_args()
+
+
+def calibrate_copy(mib):
+    """A plain copy of known size, to scale FETCH_SIZE / WRITE_SIZE readings."""
+    import torch
+    n = int(mib * (1 << 20)) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
 
 
 def cpu_baseline(budget_s):
@@ -66,6 +82,9 @@ def cpu_baseline(budget_s):
 
 def main():
     a = parse()
+    if a.calibrate_copy > 0:
+        calibrate_copy(a.calibrate_copy)
+        return
     if a.gpus != 1:
         print("bench.py runs on one GPU only: this engine has no multi-GPU path in this build "
               "(particles shard by id range; see DESIGN.md)", file=sys.stderr)
@@ -75,12 +94,15 @@ def main():
 
     n = int(a.n)
     eng = Engine(0)
-    bp = abi.make_params(abi.MODEL_1D_BP, N=n, seed=a.seed)
-    fp = abi.make_params(abi.MODEL_1D_FP, N=n, seed=a.seed + 1)
+    models = {"bp+fp": (abi.MODEL_1D_BP, abi.MODEL_1D_FP), "bp": (abi.MODEL_1D_BP,),
+              "fp": (abi.MODEL_1D_FP,), "sp": (abi.MODEL_2D_SOLARPROP,),
+              "ge": (abi.MODEL_2D_GELIOSPHERE,)}[a.model]
+    plist = [abi.make_params(m, N=n, seed=a.seed + m) for m in models]
+    dom = models[0]
 
     def step(i):
         res = []
-        for p in (bp, fp):
+        for p in plist:
             q = abi.clone_params(p, seed=p.seed + 1000 * i)
             s, _, _ = eng.run_plain(q)
             st = eng.last_stats()
@@ -97,12 +119,13 @@ def main():
 
     total_steps = sum(r[1] for r in runs)
     value = total_steps / wall
-    bp_runs = [r for r in runs if r[0] == abi.MODEL_1D_BP]
+    bp_runs = [r for r in runs if r[0] == dom]
     bp_steps = sum(r[1] for r in bp_runs) / len(bp_runs)
     bp_ms = sum(r[2] for r in bp_runs) / len(bp_runs)
-    achieved = ALG_FLOPS_PER_STEP[abi.MODEL_1D_BP] * bp_steps / (bp_ms * 1e-3) / 1e12
+    achieved = ALG_FLOPS_PER_STEP[dom] * bp_steps / (bp_ms * 1e-3) / 1e12
+    kname = {0: "Fp", 1: "Bp", 2: "Sp", 3: "Ge"}[dom]
     out = {
-        "metric": "particle-steps/s (plain MC, 1D B-p + 1D F-p)",
+        "metric": "particle-steps/s (plain MC, %s)" % a.model,
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -119,13 +142,14 @@ def main():
                    "particles_per_model": n},
         "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "plain_kernel<Bp>", "kernel_ms": bp_ms,
-                     "alg_flops_per_step": ALG_FLOPS_PER_STEP[abi.MODEL_1D_BP],
+                     "kernel": "plain_kernel<%s>" % kname, "kernel_ms": bp_ms,
+                     "alg_flops_per_step": ALG_FLOPS_PER_STEP[dom],
                      "steps_per_launch": bp_steps},
         "detail": {
-            "bp_steps_per_s_kernel": bp_steps / (bp_ms * 1e-3),
-            "fp_steps_per_launch": sum(r[1] for r in runs if r[0] == abi.MODEL_1D_FP) / len(bp_runs),
-            "spectra_per_s": 2 * a.steps / wall,
+            "dominant_steps_per_s_kernel": bp_steps / (bp_ms * 1e-3),
+            "steps_per_launch": {abi.MODEL_NAMES[m]: sum(r[1] for r in runs if r[0] == m) / a.steps
+                                 for m in models},
+            "spectra_per_s": len(models) * a.steps / wall,
         },
     }
     eng.close()

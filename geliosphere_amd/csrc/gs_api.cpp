@@ -16,6 +16,10 @@
 
 #include "../../include/gs_api.h"
 #include "gs_kernels.h"
+#include "gs_math.hpp"
+
+static_assert(sizeof(gs_params) == 680, "gs_params layout");
+static_assert(sizeof(gs_exit_record) == 48, "gs_exit_record layout");
 
 using gs::DevOut;
 using gs::DevParams;
@@ -36,9 +40,9 @@ struct gs_ctx {
     // device buffers, grown on demand
     uint64_t *d_words = nullptr;   // queue | count | digits | stats for up to cap_sims sims
     size_t cap_sims = 0;
-    gs_exit_record *d_recs = nullptr;
-    uint64_t cap_recs = 0;
-    uint64_t *d_nrec = nullptr;
+    gs::DevEvent *d_events = nullptr;
+    uint64_t cap_events = 0;
+    uint64_t *d_nev = nullptr;
     DevParams *d_params = nullptr;
     size_t cap_params = 0;
     uint32_t block_override[4] = {0, 0, 0, 0};
@@ -110,6 +114,11 @@ DevParams derive(const gs_params &p) {
     d.log_ratio = (p.T_max > 0 && p.T_min > 0) ? std::log(p.T_max / p.T_min) : 0.0;
     d.grid_step = p.grid_n ? (p.T_max - p.T_min) / p.grid_n : 0.0;
     d.sqdt = std::sqrt(p.dt);
+    d.two_dt = 2.0 * p.dt;
+    const double vs = p.model == GS_MODEL_1D_BP ? -p.V : p.V;
+    d.vdt = vs * p.dt;
+    d.cp = (2.0 / 3.0) * vs * p.dt;
+    d.csum = (4.0 / 3.0) * p.V * p.dt;
     d.omega_v = 2.866e-6 / p.V;
     d.omega2_v2 = 2.866e-6 * 2.866e-6 / (p.V * p.V);
     d.th_lo = 1.7 * kPi / 180.;
@@ -137,11 +146,11 @@ gs_status ensure_params(gs_ctx *ctx, size_t n) {
     return GS_OK;
 }
 
-gs_status ensure_recs(gs_ctx *ctx, uint64_t n) {
-    if (ctx->cap_recs >= n) return GS_OK;
-    if (ctx->d_recs) { (void)hipFree(ctx->d_recs); ctx->d_recs = nullptr; ctx->cap_recs = 0; }
-    HIP_TRY(ctx, hipMalloc((void **)&ctx->d_recs, n * sizeof(gs_exit_record)));
-    ctx->cap_recs = n;
+gs_status ensure_events(gs_ctx *ctx, uint64_t n) {
+    if (ctx->cap_events >= n) return GS_OK;
+    if (ctx->d_events) { (void)hipFree(ctx->d_events); ctx->d_events = nullptr; ctx->cap_events = 0; }
+    HIP_TRY(ctx, hipMalloc((void **)&ctx->d_events, n * sizeof(gs::DevEvent)));
+    ctx->cap_events = n;
     return GS_OK;
 }
 
@@ -220,12 +229,31 @@ void fill_spectrum(const gs_params &p, const uint64_t *count, const uint64_t *di
     out->n_acc_overflow = overflow;
 }
 
-unsigned grid_for(gs_ctx *ctx, int model, int lds_bytes) {
+unsigned grid_for(gs_ctx *ctx, int model) {
     int bpcu = (int)ctx->bpcu_override[model];
     if (bpcu <= 0) {
-        if (gs_plain_occupancy(model, lds_bytes, &bpcu) != hipSuccess || bpcu <= 0) bpcu = 1;
+        if (gs_plain_occupancy(model, &bpcu) != hipSuccess || bpcu <= 0) bpcu = 1;
     }
     return (unsigned)(bpcu * ctx->n_cu);
+}
+
+// Particles per trajectory launch: bounds the event buffer (B-p and 2D models
+// produce at most one event per particle) to 2^27 x 56 B = 7.5 GB.
+const uint64_t kMaxChunk = 1ull << 27;
+
+uint64_t event_room(int model, uint64_t n) {
+    if (model == GS_MODEL_1D_FP) return std::max<uint64_t>(n / 16, 1u << 16);
+    return std::max<uint64_t>(n, 1);
+}
+
+gs_exit_record to_record(const gs::DevEvent &e) {
+    gs_exit_record r;
+    r.t_inj = e.t_inj; r.t_exit = e.t_exit; r.r = e.r; r.theta = e.theta; r.sumac = e.sumac; r.id = e.id;
+    return r;
+}
+
+bool record_less(const gs_exit_record &a, const gs_exit_record &b) {
+    return a.id != b.id ? a.id < b.id : a.sumac < b.sumac;
 }
 
 }  // namespace
@@ -253,7 +281,7 @@ gs_status gs_create(gs_ctx **out, int device) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
-    if (e == hipSuccess) e = hipMalloc((void **)&ctx->d_nrec, sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&ctx->d_nev, sizeof(uint64_t));
     if (e != hipSuccess) {
         gs_destroy(ctx);
         return GS_ERR_HIP;
@@ -268,8 +296,8 @@ void gs_destroy(gs_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->d_words) (void)hipFree(ctx->d_words);
-    if (ctx->d_recs) (void)hipFree(ctx->d_recs);
-    if (ctx->d_nrec) (void)hipFree(ctx->d_nrec);
+    if (ctx->d_events) (void)hipFree(ctx->d_events);
+    if (ctx->d_nev) (void)hipFree(ctx->d_nev);
     if (ctx->d_params) (void)hipFree(ctx->d_params);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -338,57 +366,131 @@ gs_status gs_run_plain(gs_ctx *ctx, const gs_params *p, gs_spectrum *out, gs_exi
     const auto t_start = std::chrono::steady_clock::now();
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     gs_status st = ensure_sims(ctx, 1);
+    if (st == GS_OK) st = ensure_params(ctx, 1);
     if (st != GS_OK) return st;
     const bool want_recs = recs != nullptr && rec_cap > 0;
-    if (want_recs) {
-        st = ensure_recs(ctx, rec_cap);
-        if (st != GS_OK) return st;
-    }
-    st = ensure_params(ctx, 1);
-    if (st != GS_OK) return st;
-    const DevParams d = derive(*p);
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->d_params, &d, sizeof d, hipMemcpyHostToDevice, ctx->stream));
     Layout L = layout(ctx->d_words, 1);
     HIP_TRY(ctx, gs_launch_zero(ctx->d_words, words_per_sim(), ctx->stream));
-    HIP_TRY(ctx, gs_launch_zero(ctx->d_nrec, 1, ctx->stream));
-    DevOut O;
-    O.queue = L.queue;
-    O.g_count = L.count;
-    O.g_digits = L.digits;
-    O.g_stats = L.stats;
-    O.recs = want_recs ? ctx->d_recs : nullptr;
-    O.rec_cap = want_recs ? rec_cap : 0;
-    O.n_rec = ctx->d_nrec;
-    const int lds_bytes = gs::lds_words((int)p->n_bins, (int)p->n_weights) * (int)sizeof(uint64_t);
-    const unsigned grid = grid_for(ctx, p->model, lds_bytes);
-    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    HIP_TRY(ctx, gs_launch_plain(d, ctx->d_params, O, grid, ctx->stream));
-    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    const unsigned grid = grid_for(ctx, p->model);
+    std::vector<gs_exit_record> kept;
+    uint64_t total_events = 0;
+    double kernel_ms = 0.0;
+    const uint64_t N = p->n_particles;
+    for (uint64_t c0 = 0; c0 < N || (N == 0 && c0 == 0); c0 += kMaxChunk) {
+        gs_params q = *p;
+        q.first_particle = p->first_particle + c0;
+        q.n_particles = std::min(kMaxChunk, N - c0);
+        const DevParams d = derive(q);
+        st = ensure_events(ctx, event_room(q.model, q.n_particles));
+        if (st != GS_OK) return st;
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->d_params, &d, sizeof d, hipMemcpyHostToDevice, ctx->stream));
+        uint64_t nev = 0;
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            HIP_TRY(ctx, gs_launch_zero(L.queue, 1, ctx->stream));
+            HIP_TRY(ctx, gs_launch_zero(ctx->d_nev, 1, ctx->stream));
+            DevOut O;
+            O.queue = L.queue; O.g_count = L.count; O.g_digits = L.digits; O.g_stats = L.stats;
+            O.events = ctx->d_events; O.ev_cap = ctx->cap_events; O.n_events = ctx->d_nev;
+            HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+            HIP_TRY(ctx, gs_launch_plain(d, ctx->d_params, O, grid, ctx->stream));
+            HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+            HIP_TRY(ctx, hipMemcpyAsync(&nev, ctx->d_nev, sizeof nev, hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            float ms = 0.f;
+            HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            kernel_ms += ms;
+            if (nev <= ctx->cap_events) break;
+            // The buffer was too small (1D F-p crossings are unbounded): the
+            // trajectories are deterministic, so run the chunk again with room
+            // for every event.  Statistics of the first attempt are discarded.
+            st = ensure_events(ctx, nev + nev / 8);
+            if (st != GS_OK) return st;
+            HIP_TRY(ctx, gs_launch_zero(L.stats, gs::ST_COUNT_, ctx->stream));
+            if (attempt == 1) return fail(ctx, GS_ERR_OVERFLOW, "event buffer overflow");
+        }
+        DevOut O;
+        O.queue = L.queue; O.g_count = L.count; O.g_digits = L.digits; O.g_stats = L.stats;
+        O.events = ctx->d_events; O.ev_cap = ctx->cap_events; O.n_events = ctx->d_nev;
+        HIP_TRY(ctx, gs_launch_fold(d, ctx->d_params, O, nev, ctx->stream));
+        total_events += nev;
+        if (want_recs && nev > 0 && kept.size() < rec_cap) {
+            std::vector<gs::DevEvent> h(nev);
+            HIP_TRY(ctx, hipMemcpyAsync(h.data(), ctx->d_events, nev * sizeof(gs::DevEvent),
+                                        hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            std::vector<gs_exit_record> rs(nev);
+            for (uint64_t i = 0; i < nev; ++i) rs[i] = to_record(h[i]);
+            std::sort(rs.begin(), rs.end(), record_less);
+            for (uint64_t i = 0; i < nev && kept.size() < rec_cap; ++i) kept.push_back(rs[i]);
+        }
+        if (N == 0) break;
+    }
     std::vector<uint64_t> host(words_per_sim());
     HIP_TRY(ctx, hipMemcpyAsync(host.data(), ctx->d_words, host.size() * sizeof(uint64_t),
                                 hipMemcpyDeviceToHost, ctx->stream));
-    uint64_t nrec = 0;
-    HIP_TRY(ctx, hipMemcpyAsync(&nrec, ctx->d_nrec, sizeof nrec, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    float ms = 0.f;
-    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     Layout H = layout(host.data(), 1);
+    H.stats[gs::ST_EVENTS] = total_events;
     fill_spectrum(*p, H.count, H.digits, H.stats, out);
-    if (want_recs) {
-        const uint64_t n = std::min(nrec, rec_cap);
-        HIP_TRY(ctx, hipMemcpy(recs, ctx->d_recs, n * sizeof(gs_exit_record), hipMemcpyDeviceToHost));
-        std::sort(recs, recs + n, [](const gs_exit_record &a, const gs_exit_record &b) {
-            return a.id != b.id ? a.id < b.id : a.sumac < b.sumac;
-        });
-    }
-    if (n_recs) *n_recs = H.stats[gs::ST_EVENTS];
-    ctx->stats.kernel_ms = ms;
+    if (want_recs) std::copy(kept.begin(), kept.end(), recs);
+    if (n_recs) *n_recs = total_events;
+    ctx->stats.kernel_ms = kernel_ms;
     ctx->stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     ctx->stats.lane_cycles = H.stats[gs::ST_CYCLES] * (uint64_t)gs::kWave * gs::kCycleSteps;
     ctx->stats.active_cycles = H.stats[gs::ST_STEPS];
     ctx->stats.grid_blocks = grid;
     ctx->stats.block_threads = gs::kBlockThreads;
     if (out->n_acc_overflow) return fail(ctx, GS_ERR_OVERFLOW, "weight accumulator overflow");
+    return GS_OK;
+}
+
+gs_status gs_debug_math(gs_ctx *ctx, int32_t fn, const double *x, double *y, uint64_t n) {
+    if (!ctx || (!x && fn != GS_DBG_NORMALS) || !y) return GS_ERR_INVALID_ARG;
+    if (fn < GS_DBG_RCP || fn > GS_DBG_RAW_SQRT) return fail(ctx, GS_ERR_INVALID_ARG, "unknown fn");
+    if (!x) return GS_ERR_INVALID_ARG;
+    if (n == 0) return GS_OK;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    double *d = nullptr;
+    HIP_TRY(ctx, hipMalloc((void **)&d, 2 * n * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(d, x, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = gs_launch_debug(fn, d, d + n, n, 0, 0, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(y, d + n, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(ctx, GS_ERR_HIP, "gs_debug_math: %s", hipGetErrorString(e));
+    return GS_OK;
+}
+
+gs_status gs_debug_normals(gs_ctx *ctx, uint64_t seed, uint64_t id, uint64_t n, double *out) {
+    if (!ctx || !out) return GS_ERR_INVALID_ARG;
+    if (n == 0) return GS_OK;
+    if (n > (1ull << 32)) return fail(ctx, GS_ERR_INVALID_ARG, "n above 2^32");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    double *d = nullptr;
+    HIP_TRY(ctx, hipMalloc((void **)&d, n * sizeof(double)));
+    hipError_t e = gs_launch_debug(GS_DBG_NORMALS, nullptr, d, n, seed, id, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(ctx, GS_ERR_HIP, "gs_debug_normals: %s", hipGetErrorString(e));
+    return GS_OK;
+}
+
+gs_status gs_host_math(int32_t fn, const double *x, double *y, uint64_t n) {
+    if (!x || !y) return GS_ERR_INVALID_ARG;
+    namespace fm = gs::fm;
+    using H = fm::HostOps;
+    for (uint64_t i = 0; i < n; ++i) {
+        double s, c;
+        switch (fn) {
+        case GS_DBG_LOG: y[i] = fm::log_e<H>(x[i], 0); break;
+        case GS_DBG_EXP: y[i] = fm::exp_e<H>(x[i]); break;
+        case GS_DBG_SIN_TURN: fm::sincos_turn<H>((uint32_t)x[i], s, c); y[i] = s; break;
+        case GS_DBG_COS_TURN: fm::sincos_turn<H>((uint32_t)x[i], s, c); y[i] = c; break;
+        case GS_DBG_LOG_U32: y[i] = fm::log_u32<H>((uint32_t)x[i]); break;
+        default: return GS_ERR_UNSUPPORTED;   // rcp / rsqrt / sqrt seeds exist on the device only
+        }
+    }
     return GS_OK;
 }
 

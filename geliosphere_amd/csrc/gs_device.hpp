@@ -18,6 +18,7 @@
 
 #include "../../include/gs_api.h"
 #include "gs_kernels.h"
+#include "gs_math.hpp"
 
 namespace gs {
 
@@ -56,48 +57,24 @@ __device__ __forceinline__ void philox10(uint32_t &x0, uint32_t &x1, uint32_t &x
     }
 }
 
-// Per-lane normal stream.  Block k of particle id is
-// Philox(ctr = {id lo, id hi, k, 0}, key = {seed lo, seed hi}); its words
-// (x0,x1) and (x2,x3) give two Box-Muller pairs: u1 = (x+1) 2^-32 in (0,1],
-// u2 = x' 2^-32, z = sqrt(-2 ln u1) (cos, sin)(2 pi u2).
-struct NormalStream {
-    uint32_t x0, x1, x2, x3;
-    uint32_t blk;
-    int nidx;     // next normal within the block, 4 = exhausted
-    double zc;    // second member of the current pair
-
-    __device__ __forceinline__ void reset() {
-        blk = 0;
-        nidx = 4;
-    }
-    __device__ __forceinline__ double next(uint64_t pid, uint32_t k0, uint32_t k1) {
-        if (nidx == 4) {
-            x0 = (uint32_t)pid;
-            x1 = (uint32_t)(pid >> 32);
-            x2 = blk;
-            x3 = 0u;
-            philox10(x0, x1, x2, x3, k0, k1);
-            ++blk;
-            nidx = 0;
-        }
-        double z;
-        if ((nidx & 1) == 0) {
-            const uint32_t a = nidx == 0 ? x0 : x2;
-            const uint32_t b = nidx == 0 ? x1 : x3;
-            const double u1 = ((double)a + 1.0) * 0x1p-32;
-            const double u2 = (double)b * 0x1p-32;
-            const double rad = sqrt(-2.0 * log(u1));
-            double sn, cs;
-            sincospi(2.0 * u2, &sn, &cs);
-            z = rad * cs;
-            zc = rad * sn;
-        } else {
-            z = zc;
-        }
-        ++nidx;
-        return z;
-    }
-};
+// Normals of a particle: block k = Philox(ctr = {id lo, id hi, k, 0},
+// key = {seed lo, seed hi}); its words (x0,x1) and (x2,x3) give two
+// Box-Muller pairs, u1 = (x+1) 2^-32 in (0,1], u2 = x' 2^-32:
+// z = sqrt(-2 ln u1) (cos, sin)(2 pi u2).  Normal 4k+i is z[i].
+__device__ __forceinline__ void normal_block(uint64_t pid, uint32_t k, uint32_t k0, uint32_t k1,
+                                             double z[4]) {
+    uint32_t x0 = (uint32_t)pid, x1 = (uint32_t)(pid >> 32), x2 = k, x3 = 0u;
+    philox10(x0, x1, x2, x3, k0, k1);
+    double s, c;
+    const double ra = fm::sqrt_pos(-2.0 * fm::log_u32d(x0));
+    fm::sincos_turnd(x1, s, c);
+    z[0] = ra * c;
+    z[1] = ra * s;
+    const double rb = fm::sqrt_pos(-2.0 * fm::log_u32d(x2));
+    fm::sincos_turnd(x3, s, c);
+    z[2] = rb * c;
+    z[3] = rb * s;
+}
 
 __device__ __forceinline__ double injection_uniform(uint64_t seed, uint64_t pid) {
     uint32_t x0 = (uint32_t)pid, x1 = (uint32_t)(pid >> 32), x2 = 0u, x3 = 1u;
@@ -115,7 +92,7 @@ __constant__ double kSolarPropBins[30] = {
 __device__ __forceinline__ double injection_energy(const DevParams &P, uint64_t pid) {
     switch (P.injection) {
     case GS_INJ_LOG_UNIFORM:
-        return P.T_min * exp(injection_uniform(P.seed, pid) * P.log_ratio);
+        return P.T_min * fm::exp_fast(injection_uniform(P.seed, pid) * P.log_ratio);
     case GS_INJ_GRID:
         return P.T_min + ((double)(pid % P.grid_n) * P.grid_step);
     case GS_INJ_SOLARPROP_BINS:
@@ -127,38 +104,39 @@ __device__ __forceinline__ double injection_energy(const DevParams &P, uint64_t 
 
 // ------------------------------------------------------------------ weights
 // Reference formulas: Algorithm/src/OneDimensionBpResults.cpp:51-60,
-// OneDimensionFpResults.cpp:55-64, TwoDimensionBpResults.cpp:45-78.
+// OneDimensionFpResults.cpp:55-64, TwoDimensionBpResults.cpp:45-78;
+// x^a is exp(a ln x).
+__device__ __forceinline__ double powx(double x, double a) { return fm::exp_fast(a * fm::log_fast(x)); }
 __device__ __forceinline__ double p_si(double T) {
     const double Tkinw = T * 1e9 * kQ;
-    const double Rig = sqrt(Tkinw * (Tkinw + (2 * kT0w))) / kQ;
-    return Rig * kQ / kC;
+    const double Rig = fm::sqrt_pos(Tkinw * (Tkinw + (2 * kT0w))) * (1.0 / kQ);
+    return Rig * kQ * (1.0 / kC);
 }
 __device__ __forceinline__ double pgev2(double T) { return T * (T + 2.0 * kT0); }
 __device__ __forceinline__ double j_weber(double T) {
-    const double tem5 = 21.1 * exp(-2.8 * log(T));
-    const double tem6 = 1.0 + (5.85 * exp(-1.22 * log(T))) + (1.18 * exp(-2.54 * log(T)));
-    return tem5 / tem6;
+    const double lt = fm::log_fast(T);
+    const double tem5 = 21.1 * fm::exp_fast(-2.8 * lt);
+    const double tem6 = 1.0 + (5.85 * fm::exp_fast(-1.22 * lt)) + (1.18 * fm::exp_fast(-2.54 * lt));
+    return tem5 * fm::rcp(tem6);
 }
 __device__ __forceinline__ double j_usoskin(double T) {
     const double Tr = 0.938;
     const double tt = T + Tr;
     const double t2 = tt + Tr;
-    const double beta = sqrt(T * t2) / tt;
-    double tem = (T + 0.67) / 1.67;
-    tem = exp(-3.93 * log(tem));
-    const double j = 2.7e3 * exp(1.12 * log(T)) * tem;
-    return j / (beta * beta);
+    const double beta2 = T * t2 * fm::rcp(tt * tt);
+    const double tem = powx((T + 0.67) * (1.0 / 1.67), -3.93);
+    return 2.7e3 * powx(T, 1.12) * tem * fm::rcp(beta2);
 }
 __device__ __forceinline__ double j_burger(double T) {
-    double tem = sqrt((T * T) + (2.0 * T * 0.938));
-    const double j = 1.9 * 1.0e4 * exp(-2.78 * log(tem));
-    tem = 0.4866 * exp(-2.51 * log(tem));
-    return j / (1 + tem);
+    const double lt = fm::log_fast(fm::sqrt_pos((T * T) + (2.0 * T * 0.938)));
+    const double j = 1.9 * 1.0e4 * fm::exp_fast(-2.78 * lt);
+    const double tem = 0.4866 * fm::exp_fast(-2.51 * lt);
+    return j * fm::rcp(1 + tem);
 }
-__device__ __forceinline__ double j_powerlaw(double T) { return exp(-2.78 * log(T + 0.94)); }
+__device__ __forceinline__ double j_powerlaw(double T) { return powx(T + 0.94, -2.78); }
 __device__ __forceinline__ double f_yamada(double p) {
     const double w = (kM0 * kM0 * kC * kC * kC * kC) + (p * p * kC * kC);
-    return exp(-1.85 * log(w)) / p;
+    return powx(w, -1.85) * fm::rcp(p);
 }
 
 struct Event {
@@ -169,28 +147,29 @@ __device__ double event_weight(const DevParams &P, int kind, const Event &e) {
     if (P.model == GS_MODEL_1D_FP) {
         if (e.r < 0.3) return 0.0;
         const double jac =
-            P.injection == GS_INJ_LOG_UNIFORM ? e.t_inj * P.log_ratio / (P.T_max - P.T_min) : 1.0;
-        const double es = exp(e.sumac);
+            P.injection == GS_INJ_LOG_UNIFORM ? e.t_inj * P.log_ratio * fm::rcp(P.T_max - P.T_min) : 1.0;
+        const double es = fm::exp_fast(e.sumac);
         if (kind == GS_W_YAMADA) {
             const double p100 = p_si(e.t_inj), p1 = p_si(e.t_exit);
-            const double w0 = f_yamada(p100) / 1e45;
+            const double w0 = f_yamada(p100) * 1e-45;
             return w0 * p1 * p1 * es * jac;
         }
-        if (kind == GS_W_POWERLAW) return j_powerlaw(e.t_inj) / pgev2(e.t_inj) * pgev2(e.t_exit) * es * jac;
+        if (kind == GS_W_POWERLAW)
+            return j_powerlaw(e.t_inj) * fm::rcp(pgev2(e.t_inj)) * pgev2(e.t_exit) * es * jac;
         const double pi = p_si(e.t_inj), po = p_si(e.t_exit);
         const double j = kind == GS_W_WEBER ? j_weber(e.t_inj)
                        : kind == GS_W_USOSKIN ? j_usoskin(e.t_inj) : j_burger(e.t_inj);
-        return j / (pi * pi) * (po * po) * es * jac;
+        return j * fm::rcp(pi * pi) * (po * po) * es * jac;
     }
     if (kind == GS_W_YAMADA) {
         const double p = p_si(e.t_exit), p1 = p_si(e.t_inj);
         return f_yamada(p) * p1 * p1;
     }
-    if (kind == GS_W_POWERLAW) return j_powerlaw(e.t_exit) * pgev2(e.t_inj) / pgev2(e.t_exit);
+    if (kind == GS_W_POWERLAW) return j_powerlaw(e.t_exit) * pgev2(e.t_inj) * fm::rcp(pgev2(e.t_exit));
     const double p = p_si(e.t_exit), p1 = p_si(e.t_inj);
     const double j = kind == GS_W_WEBER ? j_weber(e.t_inj)
                    : kind == GS_W_USOSKIN ? j_usoskin(e.t_exit) : j_burger(e.t_exit);
-    return (j / (p * p)) * p1 * p1;
+    return (j * fm::rcp(p * p)) * p1 * p1;
 }
 
 // ------------------------------------------------------------------ histogram
@@ -249,24 +228,13 @@ __device__ __forceinline__ int find_bin(const DevParams &P, double x) {
     return lo;
 }
 
-// Fold one event into the workgroup histogram and, optionally, the record
-// buffer.
-__device__ void fold_event(const DevParams &P, uint64_t *lds, const Event &e, uint64_t pid,
-                           gs_exit_record *recs, uint64_t rec_cap, unsigned long long *n_rec) {
+// Fold one event into the workgroup histogram (LDS layout: count[nb] |
+// digits[nw][2][nb][16] | stat[ST_COUNT_]).
+__device__ void fold_one(const DevParams &P, uint64_t *lds, const DevEvent &e) {
     const int nb = (int)P.n_bins, nw = (int)P.n_weights;
     uint64_t *cnt = lds;
     uint64_t *dig = lds + nb;
     uint64_t *stat = dig + nw * 2 * nb * GS_ACC_LIMBS;
-    atomicAdd((unsigned long long *)&stat[ST_EVENTS], 1ull);
-    if (recs) {
-        const unsigned long long i = atomicAdd(n_rec, 1ull);
-        if (i < rec_cap) {
-            gs_exit_record r;
-            r.t_inj = e.t_inj; r.t_exit = e.t_exit; r.r = e.r; r.theta = e.theta;
-            r.sumac = e.sumac; r.id = pid;
-            recs[i] = r;
-        }
-    }
     const double x = P.model == GS_MODEL_1D_FP ? e.t_exit : e.t_inj;
     const int b = find_bin(P, x);
     if (b < 0) {
@@ -274,8 +242,10 @@ __device__ void fold_event(const DevParams &P, uint64_t *lds, const Event &e, ui
         return;
     }
     atomicAdd((unsigned long long *)&cnt[b], 1ull);
+    Event ev;
+    ev.t_inj = e.t_inj; ev.t_exit = e.t_exit; ev.r = e.r; ev.theta = e.theta; ev.sumac = e.sumac;
     for (int k = 0; k < nw; ++k) {
-        const double w = event_weight(P, P.weights[k], e);
+        const double w = event_weight(P, P.weights[k], ev);
         acc_add(dig + ((k * 2 + 0) * nb + b) * GS_ACC_LIMBS, stat, w);
         acc_add(dig + ((k * 2 + 1) * nb + b) * GS_ACC_LIMBS, stat, w * w);
     }
@@ -287,27 +257,28 @@ enum { STEP_CONTINUE = 0, STEP_EVENT = 1, STEP_EVENT_DONE = 2, STEP_DONE = 3 };
 
 // 1D B-p.  State: r [AU], P = pc/q [GV], beta (of the last computed T, which
 // a reflection does not restore, as in OneDimensionBpCpuModel.cpp:108-140).
+// Pm.vdt = -V dt and Pm.cp = -(2/3) V dt carry the backward sign of V
+// (Constants.hpp:213-218).
 struct Bp {
+    static constexpr int kNormals = 1;
     double r, P, beta, t_inj;
     __device__ __forceinline__ bool init(const DevParams &Pm, double T) {
         t_inj = T;
-        P = sqrt(T * (T + 2.0 * kT0));
-        beta = P / (T + kT0);
+        P = fm::sqrt_pos(T * (T + 2.0 * kT0));
+        beta = P * fm::rcp(T + kT0);
         r = Pm.r_inj;
         return r < 100.0002;
     }
-    __device__ __forceinline__ int step(const DevParams &Pm, NormalStream &ns, uint64_t pid,
-                                        uint32_t k0, uint32_t k1, Event &ev) {
-        const double Vb = -Pm.V;                       // Constants.hpp:213-218
-        const double K = Pm.K0 * beta * P;
-        const double inv_r = 1.0 / r;
-        const double Pn = P - (2.0 / 3.0) * Vb * Pm.dt * P * inv_r;
-        const double z = ns.next(pid, k0, k1);
-        const double dr = (Vb + 2.0 * K * inv_r) * Pm.dt + z * sqrt(2.0 * Pm.dt * K);
+    __device__ __forceinline__ int step(const DevParams &Pm, const double *z, Event &ev) {
+        const double K2dt = Pm.two_dt * (Pm.K0 * beta * P);          // 2 K dt
+        const double inv_r = fm::rcp(r);
+        const double Pn = fma(-(Pm.cp * P), inv_r, P);               // p - 2 V p dt / (3 r)
+        const double dr = fma(K2dt, inv_r, Pm.vdt) + z[0] * fm::sqrt_pos(K2dt);
         double rn = r + dr;
-        const double E = sqrt(kT0 * kT0 + Pn * Pn);
-        const double T = E - kT0;
-        const double bn = Pn / E;
+        const double E2 = fma(Pn, Pn, kT0 * kT0);
+        const double invE = fm::rsqrt(E2);
+        const double T = fma(E2, invE, -kT0);                         // sqrt(T0^2 + P^2) - T0
+        const double bn = Pn * invE;
         int out = STEP_CONTINUE;
         if (bn > kBetaMinBp && T < 200.0) {
             if (rn > 100.0 && (rn - dr) < 100.0) {
@@ -319,11 +290,9 @@ struct Bp {
             out = STEP_DONE;
         }
         beta = bn;
-        if (rn < kReflectBp) {
-            rn = rn - dr;      // P keeps its pre-step value
-        } else {
-            P = Pn;
-        }
+        const bool refl = rn < kReflectBp;
+        rn = refl ? rn - dr : rn;     // a reflected step keeps P
+        P = refl ? P : Pn;
         r = rn;
         if (!(rn < 100.0002)) out = STEP_DONE;
         return out;
@@ -331,29 +300,30 @@ struct Bp {
 };
 
 // 1D F-p.  State: r, P [GV], beta and R (rigidity) of the last computed T,
-// sumac.  OneDimensionFpCpuModel.cpp:84-152.
+// sumac.  OneDimensionFpCpuModel.cpp:84-152.  Pm.vdt = V dt,
+// Pm.cp = (2/3) V dt, Pm.csum = (4/3) V dt.
 struct Fp {
+    static constexpr int kNormals = 1;
     double r, P, beta, R, sumac, t_inj;
     __device__ __forceinline__ bool init(const DevParams &Pm, double T) {
         t_inj = T;
-        P = sqrt(T * (T + 2.0 * kT0));
+        P = fm::sqrt_pos(T * (T + 2.0 * kT0));
         R = P;
-        beta = P / (T + kT0);
+        beta = P * fm::rcp(T + kT0);
         sumac = 0.0;
         r = 100.0001;
         return true;
     }
-    __device__ __forceinline__ int step(const DevParams &Pm, NormalStream &ns, uint64_t pid,
-                                        uint32_t k0, uint32_t k1, Event &ev) {
-        const double K = Pm.K0 * beta * R;
-        const double inv_r = 1.0 / r;
-        const double z = ns.next(pid, k0, k1);
-        const double dr = (Pm.V + 2.0 * K * inv_r) * Pm.dt + z * sqrt(2.0 * Pm.dt * K);
-        const double Pn = P - (2.0 / 3.0) * Pm.V * Pm.dt * P * inv_r;
-        sumac += (4.0 / 3.0) * Pm.V * Pm.dt * inv_r;
+    __device__ __forceinline__ int step(const DevParams &Pm, const double *z, Event &ev) {
+        const double K2dt = Pm.two_dt * (Pm.K0 * beta * R);
+        const double inv_r = fm::rcp(r);
+        const double dr = fma(K2dt, inv_r, Pm.vdt) + z[0] * fm::sqrt_pos(K2dt);
+        const double Pn = fma(-(Pm.cp * P), inv_r, P);
+        sumac = fma(Pm.csum, inv_r, sumac);
         double rn = r + dr;
-        const double E = sqrt(kT0 * kT0 + Pn * Pn);
-        const double T = E - kT0;
+        const double E2 = fma(Pn, Pn, kT0 * kT0);
+        const double invE = fm::rsqrt(E2);
+        const double T = fma(E2, invE, -kT0);
         int out = STEP_CONTINUE;
         if (beta > 0.01 && T < 100.0) {
             // (r - 1) / ((r - dr) - 1) < 0, without the division
@@ -364,13 +334,11 @@ struct Fp {
             }
         }
         const bool stop = beta < 0.01;
-        beta = Pn / E;
+        beta = Pn * invE;
         R = Pn;
-        if (rn < 0.1) {
-            rn = r;            // P keeps its pre-step value
-        } else {
-            P = Pn;
-        }
+        const bool refl = rn < 0.1;
+        rn = refl ? r : rn;           // a reflected step keeps r and P
+        P = refl ? P : Pn;
         r = rn;
         if (stop || !(rn < 100.0002)) out = (out == STEP_EVENT) ? STEP_EVENT_DONE : STEP_DONE;
         return out;
@@ -380,6 +348,7 @@ struct Fp {
 // 2D SolarProp-like: SolarPropLikeCpuModel.cpp:93-256 (Kappl 2016, Eq. 13-18,
 // 26, 32-37, 42-44).
 struct Sp {
+    static constexpr int kNormals = 2;
     double r, theta, T, t_inj;
     __device__ __forceinline__ bool init(const DevParams &Pm, double Tin) {
         t_inj = Tin;
@@ -388,20 +357,19 @@ struct Sp {
         theta = Pm.theta_inj;
         return r < 100.0;
     }
-    __device__ __forceinline__ int step(const DevParams &Pm, NormalStream &ns, uint64_t pid,
-                                        uint32_t k0, uint32_t k1, Event &ev) {
+    __device__ __forceinline__ int step(const DevParams &Pm, const double *z, Event &ev) {
         double s, c;
         sincos(theta, &s, &c);
-        const double Rig = sqrt(T * (T + 2.0 * kT0));
-        const double inv_E = 1.0 / (T + kT0);
+        const double Rig = fm::sqrt_pos(T * (T + 2.0 * kT0));
+        const double inv_E = fm::rcp(T + kT0);
         const double beta = Rig * inv_E;
         const double r2 = r * r;
-        const double inv_r = 1.0 / r;
+        const double inv_r = fm::rcp(r);
         const double gamma = r * Pm.omega_v * s;
         const double g2 = gamma * gamma;
         const double opg = 1.0 + g2;
-        const double inv_opg = 1.0 / opg;
-        const double sq = sqrt(opg);
+        const double inv_opg = fm::rcp(opg);
+        const double sq = fm::sqrt_pos(opg);
         const double inv_sq = sq * inv_opg;
         const double Rc = (Rig < 0.1) ? 0.1 : Rig;
         const double Kpar = Pm.K0 * beta * Rc * (5.0 / 3.4) * r2 * inv_sq * (1.0 / 3.0);
@@ -414,25 +382,25 @@ struct Sp {
         const double t2 = (1.0 - Pm.ratio) * kbr * ((2.0 * r * opg * sq) - (1.5 * r2 * dKrrTmp * sq)) *
                           (inv_opg * inv_opg * inv_opg);
         const double dKrr = (t1 + t2) * (5.0 / (3.0 * 3.4));
-        const double z1 = ns.next(pid, k0, k1);
-        double dr = ((-Pm.V) + (2.0 * Krr * inv_r) + dKrr) * Pm.dt + z1 * sqrt(2.0 * Pm.dt * Krr);
-        const double inv_s = 1.0 / s;
-        const double z2 = ns.next(pid, k0, k1);
-        double dth = Ktt * c * inv_s * inv_r * inv_r * Pm.dt * inv_opg + z2 * sqrt(2.0 * Pm.dt * Ktt) * inv_r;
-        const double dTkin = -(2.0 / 3.0) * Pm.V * Pm.dt * ((T + 2.0 * kT0) * inv_E) * T * inv_r;
+        const double z1 = z[0];
+        double dr = ((-Pm.V) + (2.0 * Krr * inv_r) + dKrr) * Pm.dt + z1 * fm::sqrt_pos(Pm.two_dt * Krr);
+        const double inv_s = fm::rcp(s);
+        const double z2 = z[1];
+        double dth = Ktt * c * inv_s * inv_r * inv_r * Pm.dt * inv_opg + z2 * fm::sqrt_pos(Pm.two_dt * Ktt) * inv_r;
+        const double dTkin = -Pm.cp * ((T + 2.0 * kT0) * inv_E) * T * inv_r;
         // drifts
         const double Larmor = (0.0225 / kA) * Rig * r2 * inv_sq;
         const double sy = sin(Pm.tilt + Larmor * (kPi / 90.0) * inv_r);
-        const double x = sy / (kPi - sy);
+        const double x = sy * fm::rcp(kPi - sy);
         const double alphaH = acos(x);
-        const double tanH = sqrt(1.0 - x * x) / x;
+        const double tanH = fm::sqrt_pos(1.0 - x * x) * fm::rcp(x);
         const double arg = (1.0 - theta * (2.0 / kPi)) * tanH;
-        const double inv_aH = 1.0 / alphaH;
+        const double inv_aH = fm::rcp(alphaH);
         const double f = atan(arg) * inv_aH;
         const double dc = kKonvF * (2.0 / (3.0 * kA)) * Rig * beta * r * gamma * f * inv_opg * inv_opg;
         const double DriftR = Pm.polarity * dc * c * inv_s;
         const double DriftTheta = -Pm.pol_theta * dc * (2.0 + g2);
-        const double fprime = -(2.0 / kPi) * tanH * inv_aH / (1.0 + arg * arg);
+        const double fprime = -(2.0 / kPi) * tanH * inv_aH * fm::rcp(1.0 + arg * arg);
         const double DriftSheetR = Pm.polarity * kKonvF * (1.0 / (3.0 * kA)) * Rig * beta * r * gamma * fprime * inv_opg;
         dr = dr + (DriftR + DriftSheetR) * Pm.dt;
         dth = dth + DriftTheta * Pm.dt * inv_r;
@@ -457,6 +425,7 @@ struct Sp {
 // The B13 and B23 noise terms of the reference multiply exact zeros and are
 // not drawn; dr takes normals 1 and 2, dtheta normal 3.
 struct Ge {
+    static constexpr int kNormals = 3;
     double r, theta, T, t_inj;
     __device__ __forceinline__ bool init(const DevParams &Pm, double Tin) {
         t_inj = Tin;
@@ -465,17 +434,16 @@ struct Ge {
         theta = Pm.theta_inj;
         return r < 100.0;
     }
-    __device__ __forceinline__ int step(const DevParams &Pm, NormalStream &ns, uint64_t pid,
-                                        uint32_t k0, uint32_t k1, Event &ev) {
+    __device__ __forceinline__ int step(const DevParams &Pm, const double *z, Event &ev) {
         constexpr double kf = 1.0 / (3.0 * (5.0 / 3.4));   // as written in GeliosphereCpuModel.cpp:210-299
         double s, c;
         sincos(theta, &s, &c);
-        const double Rig = sqrt(T * (T + 2.0 * kT0));
-        const double inv_E = 1.0 / (T + kT0);
+        const double Rig = fm::sqrt_pos(T * (T + 2.0 * kT0));
+        const double inv_E = fm::rcp(T + kT0);
         const double beta = Rig * inv_E;
         const double r2 = r * r;
-        const double inv_r = 1.0 / r;
-        const double inv_s = 1.0 / s;
+        const double inv_r = fm::rcp(r);
+        const double inv_s = fm::rcp(s);
         const bool polar = (theta < Pm.th_lo) || (theta > Pm.th_hi);
         const bool band = (theta > Pm.th_lo) && (theta < Pm.th_hi);
         const double delta = polar ? 0.003 : Pm.delta0 * inv_s;
@@ -484,8 +452,8 @@ struct Ge {
         const double gamma = r * Pm.omega_v * s;
         const double g2 = gamma * gamma;
         const double Cb = 1.0 + g2 + r2 * deltarh2;
-        const double inv_Cb = 1.0 / Cb;
-        const double sqCb = sqrt(Cb);
+        const double inv_Cb = fm::rcp(Cb);
+        const double sqCb = fm::sqrt_pos(Cb);
         const double inv_sqCb = sqCb * inv_Cb;
         const double inv_Cb15 = inv_Cb * inv_sqCb;
         const double inv_Cb25 = inv_Cb * inv_Cb15;
@@ -496,9 +464,9 @@ struct Ge {
         const double Krr = Kper + dK * inv_Cb;
         const double Ktt = Kper + r2 * deltarh2 * dK * inv_Cb;
         const double Krt = deltarh * dK * r * inv_Cb;
-        const double inv_Ktt = 1.0 / Ktt;
-        const double sq2Ktt = sqrt(2.0 * Ktt);
-        const double B11 = sqrt(2.0 * (Krr * Ktt - Krt * Krt) * inv_Ktt);
+        const double inv_Ktt = fm::rcp(Ktt);
+        const double sq2Ktt = fm::sqrt_pos(2.0 * Ktt);
+        const double B11 = fm::sqrt_pos(2.0 * (Krr * Ktt - Krt * Krt) * inv_Ktt);
         const double B12 = Krt * inv_Ktt * sq2Ktt;
         const double B22 = sq2Ktt * inv_r;
         const double kbr = Pm.K0 * beta * Rig;
@@ -530,27 +498,27 @@ struct Ge {
         const double dKrtt = band ? -krtt0 * delta * 3.0 * g2 * c * inv_s
                                   : -krtt0 * Pm.delta0 * c * inv_s * inv_s *
                                         (1.0 - (2.0 * r2 * deltarh2) + (4.0 * g2));
-        const double z1 = ns.next(pid, k0, k1);
-        const double z2 = ns.next(pid, k0, k1);
+        const double z1 = z[0];
+        const double z2 = z[1];
         double dr = ((-Pm.V) + (2.0 * Krr * inv_r) + dKrr) * Pm.dt + dKrtt * Pm.dt * inv_r +
                     Krt * c * Pm.dt * inv_r * inv_s + (z1 * B11 + z2 * B12) * Pm.sqdt;
-        const double z3 = ns.next(pid, k0, k1);
+        const double z3 = z[2];
         double dth = Ktt * c * inv_s * inv_r * inv_r * Pm.dt + dKtt * Pm.dt * inv_r * inv_r +
                      dKrtr * Pm.dt + 2.0 * Krt * Pm.dt * inv_r + z3 * B22 * Pm.sqdt;
-        const double dTkin = -(2.0 / 3.0) * Pm.V * Pm.dt * ((T + 2.0 * kT0) * inv_E) * T * inv_r;
+        const double dTkin = -Pm.cp * ((T + 2.0 * kT0) * inv_E) * T * inv_r;
         // drifts (Kappl 2016 Eq. 26, 32-37 with Cb in place of 1 + gamma^2)
         const double Larmor = (0.0225 / kA) * Rig * r2 * inv_sqCb;
         const double sy = sin(Pm.tilt + Larmor * (kPi / 90.0) * inv_r);
-        const double x = sy / (kPi - sy);
+        const double x = sy * fm::rcp(kPi - sy);
         const double alphaH = acos(x);
-        const double tanH = sqrt(1.0 - x * x) / x;
+        const double tanH = fm::sqrt_pos(1.0 - x * x) * fm::rcp(x);
         const double arg = (1.0 - theta * (2.0 / kPi)) * tanH;
-        const double inv_aH = 1.0 / alphaH;
+        const double inv_aH = fm::rcp(alphaH);
         const double f = atan(arg) * inv_aH;
         const double dc = kKonvF * (2.0 / (3.0 * kA)) * Rig * beta * r * gamma * f * inv_Cb * inv_Cb;
         const double DriftR = Pm.polarity * dc * c * inv_s;
         const double DriftTheta = -Pm.polarity * dc * (2.0 + g2);
-        const double fprime = -(2.0 / kPi) * tanH * inv_aH / (1.0 + arg * arg);
+        const double fprime = -(2.0 / kPi) * tanH * inv_aH * fm::rcp(1.0 + arg * arg);
         const double DriftSheetR = Pm.polarity * kKonvF * (1.0 / (3.0 * kA)) * Rig * beta * r * gamma * fprime * inv_Cb;
         dr = dr + (DriftR + DriftSheetR) * Pm.dt;
         dth = dth + DriftTheta * Pm.dt * inv_r;

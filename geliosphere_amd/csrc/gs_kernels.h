@@ -40,6 +40,10 @@ struct DevParams {
     double log_ratio;     // ln(T_max / T_min)
     double grid_step;     // (T_max - T_min) / grid_n
     double sqdt;          // sqrt(dt)
+    double two_dt;        // 2 dt
+    double vdt;           // signed V dt as the model uses it (1D B-p: -V dt)
+    double cp;            // (2/3) V dt with the same sign (momentum / energy loss)
+    double csum;          // (4/3) V dt (1D F-p loss integral)
     double omega_v;       // omega / V
     double omega2_v2;     // omega^2 / V^2
     double th_lo, th_hi;  // 1.7 deg and 178.3 deg in rad, as the reference writes them
@@ -49,17 +53,25 @@ struct DevParams {
 };
 
 
+// One event as the trajectory kernels store it (exit or 1 AU crossing).
+struct DevEvent {
+    double t_inj, t_exit, r, theta, sumac;
+    uint64_t id;
+    uint32_t sim;   // simulation index within a batch (0 for plain runs)
+    uint32_t pad;
+};
+
 // Global outputs of a launch.  Per simulation: count[GS_MAX_BINS],
 // digits[GS_MAX_WEIGHTS][2][GS_MAX_BINS][GS_ACC_LIMBS] (32-bit values in
 // 64-bit words), stats[ST_COUNT_].
 struct DevOut {
-    uint64_t *queue;
+    uint64_t *queue;      // work queue (particle ids handed out)
     uint64_t *g_count;
     uint64_t *g_digits;
     uint64_t *g_stats;
-    gs_exit_record *recs;
-    uint64_t rec_cap;
-    uint64_t *n_rec;
+    DevEvent *events;     // event buffer
+    uint64_t ev_cap;
+    uint64_t *n_events;   // events produced (may exceed ev_cap: then rerun with more room)
 };
 
 // LDS layout per workgroup: count[n_bins] | digits[n_weights][2][n_bins][16] | stat[ST_COUNT_]
@@ -71,7 +83,12 @@ constexpr size_t kDigitsPerSim = (size_t)GS_MAX_WEIGHTS * 2 * GS_MAX_BINS * GS_A
 }  // namespace gs
 
 hipError_t gs_launch_zero(uint64_t *p, size_t n, hipStream_t s);
-hipError_t gs_plain_occupancy(int model, int lds_bytes, int *blocks_per_cu);
-// P: host copy (launch shape), dP: the same block in device memory.
+hipError_t gs_launch_debug(int fn, const double *x, double *y, uint64_t n, uint64_t seed,
+                           uint64_t pid, hipStream_t s);
+hipError_t gs_plain_occupancy(int model, int *blocks_per_cu);
+// Trajectories of one simulation.  P: host copy, dP: the same block in device memory.
 hipError_t gs_launch_plain(const gs::DevParams &P, const gs::DevParams *dP, const gs::DevOut &O,
                            unsigned grid, hipStream_t s);
+// Histogram of the events [0, n) of one simulation.
+hipError_t gs_launch_fold(const gs::DevParams &P, const gs::DevParams *dP, const gs::DevOut &O,
+                          uint64_t n, hipStream_t s);

@@ -16,31 +16,35 @@
 
 namespace gs {
 
-struct Lanes {
-    uint64_t pid;
-    uint32_t steps;
-    NormalStream ns;
-};
+// Append one event to the event buffer (counted even when the buffer is full).
+__device__ __forceinline__ void emit(const DevOut &O, const Event &ev, uint64_t pid, uint32_t sim) {
+    const unsigned long long i = atomicAdd((unsigned long long *)O.n_events, 1ull);
+    if (i < O.ev_cap) {
+        DevEvent d;
+        d.t_inj = ev.t_inj; d.t_exit = ev.t_exit; d.r = ev.r; d.theta = ev.theta;
+        d.sumac = ev.sumac; d.id = pid; d.sim = sim; d.pad = 0;
+        O.events[i] = d;
+    }
+}
 
 template <class Model>
-__device__ void run_lanes(const DevParams &P, uint64_t *lds, unsigned long long *queue,
-                          gs_exit_record *recs, uint64_t rec_cap, unsigned long long *n_rec) {
+__device__ void run_lanes(const DevParams &P, uint64_t *stat, unsigned long long *queue,
+                          const DevOut &O, uint32_t sim) {
+    constexpr int NPS = Model::kNormals;   // normals per step; a cycle uses NPS Philox blocks
     const int lane = (int)(threadIdx.x & (kWave - 1));
     const uint64_t lane_lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    uint64_t *stat = lds + lds_words((int)P.n_bins, (int)P.n_weights) - ST_COUNT_;
     const uint32_t k0 = (uint32_t)P.seed, k1 = (uint32_t)(P.seed >> 32);
     const uint64_t N = P.n_particles;
-    const uint64_t cap = P.max_steps;
+    const uint32_t cap = (uint32_t)P.max_steps;
 
     Model m;
-    Lanes L;
-    L.pid = 0;
-    L.steps = 0;
-    L.ns.reset();
+    uint64_t pid = 0;
+    uint32_t steps = 0, blk = 0;
     bool active = false;
     uint64_t q_next = 0, q_end = 0;
     bool exhausted = false;
-    uint64_t my_steps = 0, my_particles = 0, my_nan = 0, my_cap = 0, cycles = 0;
+    uint64_t my_steps = 0;
+    uint32_t my_particles = 0, my_nan = 0, my_cap = 0, cycles = 0;
 
     for (;;) {
         // ---- refill idle lanes (wave-uniform control flow)
@@ -72,11 +76,12 @@ __device__ void run_lanes(const DevParams &P, uint64_t *lds, unsigned long long 
                 q_next += (uint64_t)n_idle;
             }
             if (take != ~0ull) {
-                L.pid = P.first_particle + take;
-                L.steps = 0;
-                L.ns.reset();
+                my_steps += steps;
+                pid = P.first_particle + take;
+                steps = 0;
+                blk = 0;
                 ++my_particles;
-                active = m.init(P, injection_energy(P, L.pid));
+                active = m.init(P, injection_energy(P, pid));
             }
         }
         if (__ballot(active) == 0) {
@@ -84,28 +89,73 @@ __device__ void run_lanes(const DevParams &P, uint64_t *lds, unsigned long long 
             continue;
         }
         ++cycles;
-        // ---- one cycle
-#pragma unroll 1
-        for (int s = 0; s < kCycleSteps; ++s) {
-            if (active) {
-                if (L.steps >= cap) {
-                    ++my_cap;
-                    active = false;
-                } else {
-                    Event ev;
-                    const int rc = m.step(P, L.ns, L.pid, k0, k1, ev);
-                    ++L.steps;
-                    ++my_steps;
-                    if (rc == STEP_EVENT || rc == STEP_EVENT_DONE)
-                        fold_event(P, lds, ev, L.pid, recs, rec_cap, n_rec);
-                    if (rc >= STEP_EVENT_DONE) {
+        // ---- one cycle: kCycleSteps steps, NPS Philox blocks
+        if constexpr (NPS == 1) {
+            // 1D: the four normals of the cycle come from one block; every
+            // normal's position is known at compile time.
+            double z[4];
+            normal_block(pid, blk, k0, k1, z);
+#pragma unroll
+            for (int s = 0; s < kCycleSteps; ++s) {
+                if (active) {
+                    if (steps >= cap) {
+                        ++my_cap;
                         active = false;
-                        if (rc == STEP_DONE && isnan(m.r)) ++my_nan;
+                    } else {
+                        Event ev;
+                        const int rc = m.step(P, &z[s], ev);
+                        ++steps;
+                        if (rc == STEP_EVENT || rc == STEP_EVENT_DONE) emit(O, ev, pid, sim);
+                        if (rc >= STEP_EVENT_DONE) {
+                            active = false;
+                            if (rc == STEP_DONE && isnan(m.r)) ++my_nan;
+                        }
+                    }
+                }
+            }
+        } else {
+            // 2D: steps stay rolled (the step body is large); normals come from
+            // a four-entry buffer refilled from the next block when empty.  All
+            // lanes are block-aligned at the cycle start, so the refill branch
+            // is wave-uniform.
+            double b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;
+            int used = 4;
+            uint32_t nb = blk;
+#pragma unroll 1
+            for (int s = 0; s < kCycleSteps; ++s) {
+                double z[NPS];
+#pragma unroll
+                for (int i = 0; i < NPS; ++i) {
+                    if (used == 4) {
+                        double t[4];
+                        normal_block(pid, nb, k0, k1, t);
+                        b0 = t[0]; b1 = t[1]; b2 = t[2]; b3 = t[3];
+                        ++nb;
+                        used = 0;
+                    }
+                    z[i] = used == 0 ? b0 : used == 1 ? b1 : used == 2 ? b2 : b3;
+                    ++used;
+                }
+                if (active) {
+                    if (steps >= cap) {
+                        ++my_cap;
+                        active = false;
+                    } else {
+                        Event ev;
+                        const int rc = m.step(P, z, ev);
+                        ++steps;
+                        if (rc == STEP_EVENT || rc == STEP_EVENT_DONE) emit(O, ev, pid, sim);
+                        if (rc >= STEP_EVENT_DONE) {
+                            active = false;
+                            if (rc == STEP_DONE && isnan(m.r)) ++my_nan;
+                        }
                     }
                 }
             }
         }
+        blk += (uint32_t)NPS;
     }
+    my_steps += steps;
     atomicAdd((unsigned long long *)&stat[ST_STEPS], (unsigned long long)my_steps);
     atomicAdd((unsigned long long *)&stat[ST_PARTICLES], (unsigned long long)my_particles);
     if (my_nan) atomicAdd((unsigned long long *)&stat[ST_NAN], (unsigned long long)my_nan);
@@ -150,14 +200,57 @@ __device__ __forceinline__ void zero_lds(uint64_t *lds, int words) {
 template <class Model>
 __global__ void __launch_bounds__(kBlockThreads) plain_kernel(const DevParams *__restrict__ Pp,
                                                               const DevOut O) {
+    __shared__ uint64_t stat[ST_COUNT_];
+    const DevParams &P = *Pp;
+    if (threadIdx.x < ST_COUNT_) stat[threadIdx.x] = 0;
+    __syncthreads();
+    run_lanes<Model>(P, stat, (unsigned long long *)O.queue, O, 0);
+    __syncthreads();
+    if (threadIdx.x < ST_COUNT_ && stat[threadIdx.x])
+        atomicAdd((unsigned long long *)&O.g_stats[threadIdx.x], (unsigned long long)stat[threadIdx.x]);
+}
+
+// Histogram pass: every workgroup folds a strided share of the events into its
+// LDS histogram and flushes it.  Sums are exact, so the order of events in the
+// buffer does not matter.
+__global__ void __launch_bounds__(kBlockThreads) fold_kernel(const DevParams *__restrict__ Pp,
+                                                             const DevOut O, uint64_t n) {
     extern __shared__ uint64_t lds[];
     const DevParams &P = *Pp;
     zero_lds(lds, lds_words((int)P.n_bins, (int)P.n_weights));
     __syncthreads();
-    run_lanes<Model>(P, lds, (unsigned long long *)O.queue, O.recs, O.rec_cap,
-                     (unsigned long long *)O.n_rec);
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        fold_one(P, lds, O.events[i]);
     __syncthreads();
     flush_hist(P, lds, O, 0);
+}
+
+// Diagnostics: evaluate one math-layer function elementwise, or dump the
+// normals of one particle.
+__global__ void debug_kernel(int fn, const double *x, double *y, uint64_t n, uint64_t seed,
+                             uint64_t pid) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s, c, z[4];
+    switch (fn) {
+    case GS_DBG_RCP: y[i] = fm::rcp(x[i]); break;
+    case GS_DBG_RSQRT: y[i] = fm::rsqrt(x[i]); break;
+    case GS_DBG_SQRT: y[i] = fm::sqrt_pos(x[i]); break;
+    case GS_DBG_LOG: y[i] = fm::log_fast(x[i]); break;
+    case GS_DBG_EXP: y[i] = fm::exp_fast(x[i]); break;
+    case GS_DBG_SIN_TURN: fm::sincos_turnd((uint32_t)x[i], s, c); y[i] = s; break;
+    case GS_DBG_COS_TURN: fm::sincos_turnd((uint32_t)x[i], s, c); y[i] = c; break;
+    case GS_DBG_LOG_U32: y[i] = fm::log_u32d((uint32_t)x[i]); break;
+    case GS_DBG_RAW_RCP: y[i] = __builtin_amdgcn_rcp(x[i]); break;
+    case GS_DBG_RAW_RSQ: y[i] = __builtin_amdgcn_rsq(x[i]); break;
+    case GS_DBG_RAW_SQRT: y[i] = __builtin_amdgcn_sqrt(x[i]); break;
+    case GS_DBG_NORMALS:
+        normal_block(pid, (uint32_t)(i >> 2), (uint32_t)seed, (uint32_t)(seed >> 32), z);
+        y[i] = z[i & 3];
+        break;
+    default: y[i] = 0.0; break;
+    }
 }
 
 __global__ void zero_kernel(uint64_t *p, size_t n) {
@@ -180,6 +273,14 @@ static const void *plain_kernel_ptr(int model) {
     }
 }
 
+hipError_t gs_launch_debug(int fn, const double *x, double *y, uint64_t n, uint64_t seed,
+                           uint64_t pid, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(debug_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, fn, x, y, n,
+                       seed, pid);
+    return hipGetLastError();
+}
+
 hipError_t gs_launch_zero(uint64_t *p, size_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     const unsigned blocks = (unsigned)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024);
@@ -187,14 +288,24 @@ hipError_t gs_launch_zero(uint64_t *p, size_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t gs_plain_occupancy(int model, int lds_bytes, int *blocks_per_cu) {
+hipError_t gs_plain_occupancy(int model, int *blocks_per_cu) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, plain_kernel_ptr(model),
-                                                        kBlockThreads, (size_t)lds_bytes);
+                                                        kBlockThreads, 0);
+}
+
+hipError_t gs_launch_fold(const DevParams &P, const DevParams *dP, const DevOut &O, uint64_t n,
+                          hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const size_t lds = (size_t)lds_words((int)P.n_bins, (int)P.n_weights) * sizeof(uint64_t);
+    uint64_t blocks = (n + kBlockThreads - 1) / kBlockThreads;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(fold_kernel, dim3((unsigned)blocks), dim3(kBlockThreads), lds, s, dP, O, n);
+    return hipGetLastError();
 }
 
 hipError_t gs_launch_plain(const DevParams &P, const DevParams *dP, const DevOut &O, unsigned grid,
                            hipStream_t s) {
-    const size_t lds = (size_t)lds_words((int)P.n_bins, (int)P.n_weights) * sizeof(uint64_t);
+    const size_t lds = 0;
     switch (P.model) {
     case GS_MODEL_1D_FP:
         hipLaunchKernelGGL(plain_kernel<Fp>, dim3(grid), dim3(kBlockThreads), lds, s, dP, O);

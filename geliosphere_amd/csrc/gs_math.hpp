@@ -1,0 +1,197 @@
+// gs_math.hpp - FP64 elementary functions for the trajectory kernels.
+//
+// The library versions of div, sqrt, log, exp, sin, cos round correctly or
+// handle every range (Payne-Hanek reduction, denormal scaling); the step
+// kernels only need ~1 ulp on known ranges.  These versions:
+//   rcp, rsqrt, sqrt  hardware seed (v_rcp_f64 / v_rsq_f64) + Newton steps;
+//   log               exponent split, s = (m-1)/(m+1), 2s + 2s w R(w);
+//   exp               k ln2 + r reduction (two-part ln2), Taylor to r^13;
+//   sincos_turn       sin/cos(2 pi x 2^-32) for a 32-bit integer x, with
+//                     exact integer quadrant reduction (Box-Muller angle);
+//   log_u32           ln((x + 1) 2^-32), the Box-Muller radius argument.
+// R, S, C are Chebyshev fits made by gen_poly.py (worst relative error of the
+// function value 4.5e-18, 1.4e-17, 6.9e-19 before rounding).
+//
+// Every multiply-add is an explicit fused multiply-add (Ops::fma3 / fmac: on
+// the device v_fma_f64, on the host std::fma).  Both are correctly rounded,
+// so the polynomial parts give the same bits on host and device; tests check
+// both against mpmath through gs_host_math and gs_debug_math.  No inline
+// assembly: the compiler's hazard recognizer must see every VALU instruction
+// (a v_rcp_f64 / v_rsq_f64 result read by the next instruction needs wait
+// states on gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cmath>
+
+#pragma clang fp contract(off)
+
+namespace gs {
+namespace fm {
+
+constexpr double kLn2Hi = 6.93147180369123816490e-01;  // upper 32 bits of ln 2
+constexpr double kLn2Lo = 1.90821492927058770002e-10;  // ln 2 - kLn2Hi
+constexpr double kInvLn2 = 1.44269504088896338700e+00;
+constexpr double kHalfPiOver2p30 = 1.5707963267948966 / 1073741824.0;  // (pi/2) 2^-30
+
+__host__ __device__ inline double bits_to_double(uint64_t b) {
+    union { uint64_t u; double d; } v;
+    v.u = b;
+    return v.d;
+}
+__host__ __device__ inline uint64_t double_to_bits(double d) {
+    union { uint64_t u; double d; } v;
+    v.d = d;
+    return v.u;
+}
+
+// 1/x for normal x: v_rcp_f64 seed + 2 Newton steps.
+__device__ __forceinline__ double rcp(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(e, y, y);
+    e = fma(-x, y, 1.0);
+    return fma(e, y, y);
+}
+
+// 1/sqrt(x) for normal x > 0: v_rsq_f64 seed + 2 Newton steps.
+__device__ __forceinline__ double rsqrt(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double r = fma(-(x * y), y, 1.0);
+    y = fma(0.5 * y, r, y);
+    r = fma(-(x * y), y, 1.0);
+    return fma(0.5 * y, r, y);
+}
+
+// sqrt(x) for x >= 0 (0 -> 0): s = x rsqrt(x) with one Heron correction.
+__device__ __forceinline__ double sqrt_pos(double x) {
+    const double y = rsqrt(x);
+    double s = x * y;
+    const double e = fma(-s, s, x);
+    s = fma(e, 0.5 * y, s);
+    return x > 0.0 ? s : 0.0;
+}
+
+// Ops policy: fma3 (general), fmac (addend a constant: Horner steps),
+// inv (reciprocal).
+struct DevOps {
+    __device__ static double fma3(double a, double b, double c) { return fma(a, b, c); }
+    __device__ static double fmac(double a, double b, double c) { return fma(a, b, c); }
+    __device__ static double inv(double v) { return rcp(v); }
+};
+struct HostOps {
+    static double fma3(double a, double b, double c) { return std::fma(a, b, c); }
+    static double fmac(double a, double b, double c) { return std::fma(a, b, c); }
+    static double inv(double v) { return 1.0 / v; }
+};
+
+// ln((1+s)/(1-s)) = 2s + 2s w R(w), w = s^2 <= 0.0295.
+template <class Ops>
+__host__ __device__ inline double log_series(double s, double w) {
+    double p = 0.07308224842521702930211;
+    p = Ops::fmac(p, w, 0.07665860800278020634313);
+    p = Ops::fmac(p, w, 0.09091444562630861040134);
+    p = Ops::fmac(p, w, 0.1111110556739753992754);
+    p = Ops::fmac(p, w, 0.1428571431298774246073);
+    p = Ops::fmac(p, w, 0.199999999999497522449);
+    p = Ops::fmac(p, w, 0.3333333333333334843082);
+    const double s2 = 2.0 * s;
+    return Ops::fma3(s2 * w, p, s2);
+}
+
+// ln(x 2^e_off) for a positive normal double x.  The power of two is folded
+// into the exponent, so ln(2^k) is exactly k ln2 (0 for k = 0).
+template <class Ops>
+__host__ __device__ inline double log_e(double x, int e_off) {
+    const uint64_t b = double_to_bits(x);
+    int e = (int)((b >> 52) & 0x7ff) - 1023 + e_off;
+    uint64_t mb = (b & 0x000fffffffffffffull) | 0x3ff0000000000000ull;  // m in [1, 2)
+    if (mb > 0x3ff6a09e667f3bcdull) {   // m > sqrt(2): use m / 2
+        mb -= 0x0010000000000000ull;
+        e += 1;
+    }
+    const double m = bits_to_double(mb);
+    const double s = (m - 1.0) * Ops::inv(m + 1.0);
+    const double lm = log_series<Ops>(s, s * s);
+    const double fe = (double)e;
+    return Ops::fmac(fe, kLn2Hi, Ops::fma3(fe, kLn2Lo, lm));
+}
+
+// exp(x) for |x| < 700 (results are normal numbers).  Taylor 1/k!, k <= 13,
+// on |r| <= ln2/2.
+template <class Ops>
+__host__ __device__ inline double exp_e(double x) {
+    const double k = rint(x * kInvLn2);
+    const double r = Ops::fma3(-k, kLn2Lo, Ops::fma3(-k, kLn2Hi, x));
+    double p = 1.0 / 6227020800.0;
+    p = Ops::fmac(p, r, 1.0 / 479001600.0);
+    p = Ops::fmac(p, r, 1.0 / 39916800.0);
+    p = Ops::fmac(p, r, 1.0 / 3628800.0);
+    p = Ops::fmac(p, r, 1.0 / 362880.0);
+    p = Ops::fmac(p, r, 1.0 / 40320.0);
+    p = Ops::fmac(p, r, 1.0 / 5040.0);
+    p = Ops::fmac(p, r, 1.0 / 720.0);
+    p = Ops::fmac(p, r, 1.0 / 120.0);
+    p = Ops::fmac(p, r, 1.0 / 24.0);
+    p = Ops::fmac(p, r, 1.0 / 6.0);
+    p = Ops::fmac(p, r, 0.5);
+    p = Ops::fmac(p, r, 1.0);
+    p = Ops::fmac(p, r, 1.0);
+    const int64_t ki = (int64_t)k;
+    return bits_to_double(double_to_bits(p) + ((uint64_t)ki << 52));
+}
+
+// sin a = a - a w S(w), cos a = 1 - w/2 + w^2 C(w), w = a^2, |a| <= pi/4.
+template <class Ops>
+__host__ __device__ inline void sincos_poly(double a, double &s, double &c) {
+    const double w = a * a;
+    double ps = -1.591812929486660866682e-10;
+    ps = Ops::fmac(ps, w, 2.505113184500362441675e-8);
+    ps = Ops::fmac(ps, w, -0.000002755731610255244011882);
+    ps = Ops::fmac(ps, w, 0.0001984126983675857432297);
+    ps = Ops::fmac(ps, w, -0.008333333333330948485545);
+    ps = Ops::fmac(ps, w, 0.1666666666666666462354);
+    s = Ops::fma3(-(a * w), ps, a);
+    double pc = -1.138263242552171800541e-11;
+    pc = Ops::fmac(pc, w, 2.087614626840319789335e-9);
+    pc = Ops::fmac(pc, w, -2.755731727172979288199e-7);
+    pc = Ops::fmac(pc, w, 0.00002480158729876568792727);
+    pc = Ops::fmac(pc, w, -0.001388888888888739723669);
+    pc = Ops::fmac(pc, w, 0.04166666666666666538875);
+    c = Ops::fma3(w * w, pc, Ops::fma3(-0.5, w, 1.0));
+}
+
+// sin/cos(2 pi x 2^-32): quadrant k = round(x / 2^30) mod 4, remainder
+// d = x - k 2^30 in [-2^29, 2^29) exactly, angle a = d (pi/2) 2^-30.
+template <class Ops>
+__host__ __device__ inline void sincos_turn(uint32_t x, double &s, double &c) {
+    const uint32_t k = ((x + 0x20000000u) >> 30) & 3u;
+    const int32_t d = (int32_t)(x - (k << 30));
+    const double a = (double)d * kHalfPiOver2p30;
+    double sa, ca;
+    sincos_poly<Ops>(a, sa, ca);
+    // sin/cos(k pi/2 + a)
+    const double ss = (k & 1u) ? ca : sa;
+    const double cc = (k & 1u) ? sa : ca;
+    s = (k & 2u) ? -ss : ss;
+    c = ((k + 1u) & 2u) ? -cc : cc;
+}
+
+// ln((x + 1) 2^-32) for a 32-bit integer x, in [-22.18, 0].
+template <class Ops>
+__host__ __device__ inline double log_u32(uint32_t x) {
+    return log_e<Ops>((double)x + 1.0, -32);   // (double)x + 1 is exact
+}
+
+// The device forms used by the kernels.
+__device__ __forceinline__ double log_fast(double x) { return log_e<DevOps>(x, 0); }
+__device__ __forceinline__ double exp_fast(double x) { return exp_e<DevOps>(x); }
+__device__ __forceinline__ double log_u32d(uint32_t x) { return log_u32<DevOps>(x); }
+__device__ __forceinline__ void sincos_turnd(uint32_t x, double &s, double &c) {
+    sincos_turn<DevOps>(x, s, c);
+}
+
+}  // namespace fm
+}  // namespace gs

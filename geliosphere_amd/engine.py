@@ -15,7 +15,10 @@ _lib = None
 
 EXPORTS = ("gs_abi_version", "gs_create", "gs_destroy", "gs_last_error", "gs_default_params",
            "gs_validate_params", "gs_run_plain", "gs_run_batch", "gs_set_launch",
-           "gs_last_run_stats")
+           "gs_last_run_stats", "gs_debug_math", "gs_debug_normals", "gs_host_math")
+
+DBG_RCP, DBG_RSQRT, DBG_SQRT, DBG_LOG, DBG_EXP, DBG_SIN_TURN, DBG_COS_TURN, DBG_LOG_U32, \
+    DBG_RAW_RCP, DBG_RAW_RSQ, DBG_RAW_SQRT = range(11)
 
 
 class GsError(RuntimeError):
@@ -53,8 +56,26 @@ def lib():
         L.gs_set_launch.restype = ctypes.c_int
         L.gs_last_run_stats.argtypes = [vp, P(abi.GsRunStats)]
         L.gs_last_run_stats.restype = ctypes.c_int
+        dp = P(ctypes.c_double)
+        L.gs_debug_math.argtypes = [vp, ctypes.c_int32, dp, dp, ctypes.c_uint64]
+        L.gs_debug_math.restype = ctypes.c_int
+        L.gs_debug_normals.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, dp]
+        L.gs_debug_normals.restype = ctypes.c_int
+        L.gs_host_math.argtypes = [ctypes.c_int32, dp, dp, ctypes.c_uint64]
+        L.gs_host_math.restype = ctypes.c_int
         _lib = L
     return _lib
+
+
+def host_math(fn, xs):
+    """Host evaluation of the math layer's polynomial parts (no device)."""
+    n = len(xs)
+    x = (ctypes.c_double * n)(*xs)
+    y = (ctypes.c_double * n)()
+    rc = lib().gs_host_math(fn, x, y, n)
+    if rc:
+        raise GsError(rc, "gs_host_math")
+    return list(y)
 
 
 def validate(p):
@@ -130,6 +151,18 @@ class Engine:
 
     def set_launch(self, model, block_threads=0, blocks_per_cu=0):
         self._check(lib().gs_set_launch(self._ctx, model, block_threads, blocks_per_cu), "gs_set_launch")
+
+    def debug_math(self, fn, xs):
+        n = len(xs)
+        x = (ctypes.c_double * n)(*xs)
+        y = (ctypes.c_double * n)()
+        self._check(lib().gs_debug_math(self._ctx, fn, x, y, n), "gs_debug_math")
+        return list(y)
+
+    def debug_normals(self, seed, pid, n):
+        y = (ctypes.c_double * n)()
+        self._check(lib().gs_debug_normals(self._ctx, seed, pid, n, y), "gs_debug_normals")
+        return list(y)
 
     def last_stats(self):
         st = abi.GsRunStats()

@@ -205,6 +205,34 @@ gs_status gs_set_launch(gs_ctx *ctx, int32_t model, uint32_t block_threads,
 /* Statistics of the last run on ctx. */
 gs_status gs_last_run_stats(const gs_ctx *ctx, gs_run_stats *st);
 
+/* ------------------------------------------------------------------------
+ * Diagnostics.  No counterpart in the reference: they expose the engine's
+ * FP64 math layer and normal stream so that tests can pin them (to the ulp
+ * against high-precision references, and against the oracle's stream).
+ * ------------------------------------------------------------------------ */
+typedef enum gs_debug_fn {
+    GS_DBG_RCP = 0,       /* 1/x                                       */
+    GS_DBG_RSQRT = 1,     /* 1/sqrt(x)                                 */
+    GS_DBG_SQRT = 2,      /* sqrt(x), x >= 0                           */
+    GS_DBG_LOG = 3,       /* ln x                                      */
+    GS_DBG_EXP = 4,       /* exp x                                     */
+    GS_DBG_SIN_TURN = 5,  /* sin(2 pi k 2^-32), k = (uint32) x         */
+    GS_DBG_COS_TURN = 6,  /* cos(2 pi k 2^-32)                         */
+    GS_DBG_LOG_U32 = 7,   /* ln((k + 1) 2^-32), k = (uint32) x         */
+    GS_DBG_RAW_RCP = 8,   /* hardware v_rcp_f64 seed alone             */
+    GS_DBG_RAW_RSQ = 9,   /* hardware v_rsq_f64 seed alone             */
+    GS_DBG_RAW_SQRT = 10, /* hardware v_sqrt_f64 alone                 */
+    GS_DBG_NORMALS = 100  /* internal: normal stream (gs_debug_normals) */
+} gs_debug_fn;
+
+/* y[i] = fn(x[i]) evaluated by the device code of the math layer. */
+gs_status gs_debug_math(gs_ctx *ctx, int32_t fn, const double *x, double *y, uint64_t n);
+/* The first n normals of particle `id` under `seed`, as the kernels draw them. */
+gs_status gs_debug_normals(gs_ctx *ctx, uint64_t seed, uint64_t id, uint64_t n, double *out);
+/* Host evaluation of the polynomial parts of the math layer (GS_DBG_LOG,
+ * _EXP, _SIN_TURN, _COS_TURN, _LOG_U32); no device needed. */
+gs_status gs_host_math(int32_t fn, const double *x, double *y, uint64_t n);
+
 #ifdef __cplusplus
 }
 #endif

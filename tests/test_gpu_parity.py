@@ -150,11 +150,18 @@ def test_fp_statistical_parity(engine, oracle):
     assert abs(rg - ro) < 3.0 * se
 
 
-def test_replay_is_bit_exact(engine):
-    p = abi.make_params(abi.MODEL_1D_BP, N=300_000, seed=5, weights=ALL_WEIGHTS)
-    a, _, _ = engine.run_plain(p)
-    b, _, _ = engine.run_plain(p)
+REPLAY_N = {abi.MODEL_1D_BP: 300_000, abi.MODEL_1D_FP: 2_000_000,
+            abi.MODEL_2D_SOLARPROP: 20_000, abi.MODEL_2D_GELIOSPHERE: 20_000}
+
+
+@pytest.mark.parametrize("model", MODELS)
+def test_replay_is_bit_exact(engine, model):
+    """Same seed, same launch: spectra and every event identical, bit for bit."""
+    p = abi.make_params(model, N=REPLAY_N[model], seed=5, weights=ALL_WEIGHTS)
+    a, ra, na = engine.run_plain(p, records=True, rec_cap=200_000)
+    b, rb, nb = engine.run_plain(p, records=True, rec_cap=200_000)
     assert bytes(a) == bytes(b)
+    assert na == nb and [bytes(x) for x in ra] == [bytes(x) for x in rb]
 
 
 @pytest.mark.parametrize("model", [abi.MODEL_1D_BP, abi.MODEL_2D_GELIOSPHERE])

@@ -1,0 +1,71 @@
+"""Accuracy of the engine's FP64 math layer (geliosphere_amd/csrc/gs_math.hpp)
+against mpmath at 50 digits: host evaluation of the polynomial parts (CPU)
+and device evaluation of everything, Newton-refined seeds included (GPU)."""
+import math
+import random
+
+import mpmath
+import pytest
+
+from geliosphere_amd import engine
+
+mpmath.mp.dps = 50
+E = engine
+
+
+def ulps(y, ref):
+    ref = float(ref)
+    if ref == 0.0:
+        return abs(y) / 5e-324
+    return abs(y - ref) / math.ulp(ref)
+
+
+def samples(seed=1):
+    rng = random.Random(seed)
+    log_x = [rng.uniform(0.5, 2.0) for _ in range(300)] + \
+            [10.0 ** rng.uniform(-300, 300) for _ in range(700)] + [1.0, 2.0, math.sqrt(2), 1e-300]
+    exp_x = [rng.uniform(-700, 700) for _ in range(600)] + [rng.uniform(-1, 1) for _ in range(400)] + [0.0]
+    u32 = [float(rng.getrandbits(32)) for _ in range(1000)] + \
+          [0.0, 1.0, 2.0 ** 29, 2.0 ** 29 - 1, 2.0 ** 30, 2.0 ** 31, 3 * 2.0 ** 30, 2.0 ** 32 - 1]
+    return log_x, exp_x, u32
+
+
+def check(fn_eval, log_x, exp_x, u32):
+    y = fn_eval(E.DBG_LOG, log_x)
+    assert max(ulps(a, mpmath.log(x)) for a, x in zip(y, log_x)) <= 2.0
+    y = fn_eval(E.DBG_EXP, exp_x)
+    assert max(ulps(a, mpmath.exp(x)) for a, x in zip(y, exp_x)) <= 2.0
+    for fn, f in ((E.DBG_SIN_TURN, mpmath.sin), (E.DBG_COS_TURN, mpmath.cos)):
+        y = fn_eval(fn, u32)
+        err = max(abs(a - float(f(2 * mpmath.pi * mpmath.mpf(int(k)) / 2 ** 32))) for a, k in zip(y, u32))
+        assert err <= 2.5e-16
+    y = fn_eval(E.DBG_LOG_U32, u32)
+    for a, k in zip(y, u32):
+        ref = mpmath.log((mpmath.mpf(int(k)) + 1) / 2 ** 32)
+        assert abs(a - float(ref)) <= 2 * math.ulp(float(ref)) + 1e-300
+    assert y[u32.index(2.0 ** 32 - 1)] == 0.0    # u1 = 1 gives exactly ln 1 = 0
+
+
+def test_host_math_accuracy():
+    check(E.host_math, *samples(1))
+
+
+@pytest.mark.gpu
+def test_device_math_accuracy(engine):
+    log_x, exp_x, u32 = samples(2)
+    check(engine.debug_math, log_x, exp_x, u32)
+    rng = random.Random(3)
+    xs = [10.0 ** rng.uniform(-200, 200) for _ in range(1000)] + [0.3, 1.0, 99.9999]
+    for fn, f in ((E.DBG_RCP, lambda v: 1 / mpmath.mpf(v)), (E.DBG_RSQRT, lambda v: 1 / mpmath.sqrt(v)),
+                  (E.DBG_SQRT, mpmath.sqrt)):
+        y = engine.debug_math(fn, xs)
+        assert max(ulps(a, f(x)) for a, x in zip(y, xs)) <= 1.0
+    assert engine.debug_math(E.DBG_SQRT, [0.0]) == [0.0]
+
+
+@pytest.mark.gpu
+def test_device_normals_match_oracle(engine, oracle):
+    for seed, pid in ((1, 0), (0xDEADBEEF12345678, 987654321), (7, 2 ** 40 + 3)):
+        z = engine.debug_normals(seed, pid, 64)
+        for n, v in enumerate(z):
+            assert v == pytest.approx(oracle.normal(seed, pid, n), abs=2e-15)
