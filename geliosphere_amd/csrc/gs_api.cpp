@@ -42,9 +42,19 @@ struct gs_ctx {
     size_t cap_sims = 0;
     gs::DevEvent *d_events = nullptr;
     uint64_t cap_events = 0;
-    uint64_t *d_nev = nullptr;
+    uint64_t *d_meta = nullptr;    // event offsets/caps/counters, id offsets, queues
+    size_t cap_meta = 0;
+    std::vector<uint64_t> h_meta;  // host staging of d_meta (outlives the async copy)
     DevParams *d_params = nullptr;
     size_t cap_params = 0;
+    std::vector<DevParams> h_params;
+    // batched runs: side streams for concurrent model groups, fork/join
+    // events, and the graph of the last batch layout
+    hipStream_t side[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t fork = nullptr, join[3] = {nullptr, nullptr, nullptr};
+    std::vector<uint64_t> graph_key;
+    hipGraphExec_t graph_exec = nullptr;
+    uint64_t batch_event_budget = 1ull << 29;
     uint32_t block_override[4] = {0, 0, 0, 0};
     uint32_t bpcu_override[4] = {0, 0, 0, 0};
     gs_run_stats stats{};
@@ -146,6 +156,14 @@ gs_status ensure_params(gs_ctx *ctx, size_t n) {
     return GS_OK;
 }
 
+gs_status ensure_meta(gs_ctx *ctx, size_t words) {
+    if (ctx->cap_meta >= words) return GS_OK;
+    if (ctx->d_meta) { (void)hipFree(ctx->d_meta); ctx->d_meta = nullptr; ctx->cap_meta = 0; }
+    HIP_TRY(ctx, hipMalloc((void **)&ctx->d_meta, words * sizeof(uint64_t)));
+    ctx->cap_meta = words;
+    return GS_OK;
+}
+
 gs_status ensure_events(gs_ctx *ctx, uint64_t n) {
     if (ctx->cap_events >= n) return GS_OK;
     if (ctx->d_events) { (void)hipFree(ctx->d_events); ctx->d_events = nullptr; ctx->cap_events = 0; }
@@ -229,12 +247,21 @@ void fill_spectrum(const gs_params &p, const uint64_t *count, const uint64_t *di
     out->n_acc_overflow = overflow;
 }
 
-unsigned grid_for(gs_ctx *ctx, int model) {
+unsigned grid_for(gs_ctx *ctx, int model, bool batch) {
     int bpcu = (int)ctx->bpcu_override[model];
     if (bpcu <= 0) {
-        if (gs_plain_occupancy(model, &bpcu) != hipSuccess || bpcu <= 0) bpcu = 1;
+        if (gs_traj_occupancy(model, batch, &bpcu) != hipSuccess || bpcu <= 0) bpcu = 1;
     }
     return (unsigned)(bpcu * ctx->n_cu);
+}
+
+size_t fold_lds_bytes(uint32_t nb, uint32_t nw) {
+    return (size_t)gs::lds_words((int)nb, (int)nw) * sizeof(uint64_t);
+}
+
+unsigned fold_blocks(uint64_t events) {
+    const uint64_t b = (events + gs::kBlockThreads - 1) / gs::kBlockThreads;
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(b, 1024));
 }
 
 // Particles per trajectory launch: bounds the event buffer (B-p and 2D models
@@ -254,6 +281,207 @@ gs_exit_record to_record(const gs::DevEvent &e) {
 
 bool record_less(const gs_exit_record &a, const gs_exit_record &b) {
     return a.id != b.id ? a.id < b.id : a.sumac < b.sumac;
+}
+
+// Relative work per particle used to order a batch (heuristic only: it
+// affects the order of work, never the results).  Diffusive time across
+// the heliosphere ~ L^2 / (K0 dt), times the per-step cost of the model.
+double est_cost(const gs_params &p) {
+    static const double step_cost[4] = {1.0, 1.0, 4.0, 6.5};
+    const double L = std::max(100.0 - std::min(p.r_inj, 100.0), 1.0);
+    const double K = std::max(p.K0, 1e-12);
+    return step_cost[p.model] * L * L / (K * p.dt);
+}
+
+const size_t kMaxBatchSims = 65535;   // fold grid.y limit
+
+gs_status run_sub_batch(gs_ctx *ctx, const gs_params *sims, const std::vector<uint32_t> &part,
+                        gs_spectrum *outs, std::vector<uint32_t> &redo, gs_run_stats &acc);
+
+}  // namespace
+
+namespace {
+
+// One sub-batch: the simulations of `part` (indices into sims), grouped by
+// model (groups of larger total work first, costlier particles first within
+// a group), one persistent launch per <= kBatchMaxSims simulations, the model
+// groups on concurrent streams, then one fold launch over all simulations.
+// The launch sequence is captured into a hipGraph once per layout and
+// replayed while the layout (counts, grids, buffers) stays the same.
+gs_status run_sub_batch(gs_ctx *ctx, const gs_params *sims, const std::vector<uint32_t> &part,
+                        gs_spectrum *outs, std::vector<uint32_t> &redo, gs_run_stats &acc) {
+    const size_t m = part.size();
+    // ---- order
+    std::vector<uint32_t> order(part);
+    double gcost[4] = {0, 0, 0, 0};
+    for (uint32_t i : order) gcost[sims[i].model] += est_cost(sims[i]) * (double)sims[i].n_particles;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        const int ma = sims[a].model, mb = sims[b].model;
+        if (ma != mb) return gcost[ma] != gcost[mb] ? gcost[ma] > gcost[mb] : ma < mb;
+        return est_cost(sims[a]) > est_cost(sims[b]);
+    });
+    // ---- launches: (model, first sim, count)
+    struct Launch { int model; uint32_t base, count; unsigned grid; size_t off_pos; };
+    std::vector<Launch> launches;
+    for (size_t j = 0; j < m;) {
+        size_t k = j;
+        while (k < m && sims[order[k]].model == sims[order[j]].model && k - j < (size_t)gs::kBatchMaxSims) ++k;
+        Launch L;
+        L.model = sims[order[j]].model;
+        L.base = (uint32_t)j;
+        L.count = (uint32_t)(k - j);
+        L.grid = grid_for(ctx, L.model, true);
+        L.off_pos = 0;
+        launches.push_back(L);
+        j = k;
+    }
+    const size_t nl = launches.size();
+    // ---- host staging: params, meta = ev_off[m] | ev_cap[m] | n_events[m] |
+    //      offsets (count+1 per launch) | queues[nl] | cycles[nl]
+    ctx->h_params.resize(m);
+    uint32_t max_nb = 1, max_nw = 1;
+    uint64_t total_room = 0;
+    std::vector<uint64_t> room(m);
+    for (size_t j = 0; j < m; ++j) {
+        const gs_params &p = sims[order[j]];
+        ctx->h_params[j] = derive(p);
+        room[j] = event_room(p.model, p.n_particles);
+        total_room += room[j];
+        max_nb = std::max(max_nb, p.n_bins);
+        max_nw = std::max(max_nw, p.n_weights);
+    }
+    const size_t off_base = 3 * m;
+    size_t pos = off_base;
+    for (Launch &L : launches) { L.off_pos = pos; pos += L.count + 1; }
+    const size_t q_base = pos, c_base = pos + nl, meta_words = pos + 2 * nl;
+    ctx->h_meta.assign(meta_words, 0);
+    uint64_t eo = 0;
+    for (size_t j = 0; j < m; ++j) {
+        ctx->h_meta[j] = eo;
+        ctx->h_meta[m + j] = room[j];
+        eo += room[j];
+    }
+    for (const Launch &L : launches) {
+        uint64_t v = 0;
+        for (uint32_t t = 0; t < L.count; ++t) {
+            ctx->h_meta[L.off_pos + t] = v;
+            v += sims[order[L.base + t]].n_particles;
+        }
+        ctx->h_meta[L.off_pos + L.count] = v;
+    }
+    gs_status st = ensure_sims(ctx, m);
+    if (st == GS_OK) st = ensure_params(ctx, m);
+    if (st == GS_OK) st = ensure_events(ctx, std::max<uint64_t>(total_room, 1));
+    if (st == GS_OK) st = ensure_meta(ctx, std::max<size_t>(meta_words, 64));
+    if (st != GS_OK) return st;
+    const Layout W = layout(ctx->d_words, m);
+    const size_t lds = fold_lds_bytes(max_nb, max_nw);
+    uint64_t max_room = 1;
+    for (uint64_t r : room) max_room = std::max(max_room, r);
+    const unsigned fb = std::max(1u, std::min(fold_blocks(max_room), (unsigned)std::max<size_t>(1, 4096 / m)));
+    // ---- graph (re)build when the layout differs from the cached one
+    std::vector<uint64_t> key = {m, nl, (uint64_t)(uintptr_t)ctx->d_words, (uint64_t)(uintptr_t)ctx->d_events,
+                                 (uint64_t)(uintptr_t)ctx->d_params, (uint64_t)(uintptr_t)ctx->d_meta, lds, fb};
+    for (const Launch &L : launches) {
+        key.push_back((uint64_t)L.model);
+        key.push_back(L.count);
+        key.push_back(L.grid);
+    }
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->d_params, ctx->h_params.data(), m * sizeof(DevParams),
+                                hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->d_meta, ctx->h_meta.data(), meta_words * sizeof(uint64_t),
+                                hipMemcpyHostToDevice, ctx->stream));
+    if (!ctx->graph_exec || key != ctx->graph_key) {
+        if (ctx->graph_exec) { (void)hipGraphExecDestroy(ctx->graph_exec); ctx->graph_exec = nullptr; }
+        ctx->graph_key.clear();
+        hipStream_t s0 = ctx->stream;
+        HIP_TRY(ctx, hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+        hipError_t e = gs_launch_zero(ctx->d_words, m * words_per_sim(), s0);
+        if (e == hipSuccess) e = gs_launch_zero(ctx->d_meta + 2 * m, m, s0);          // n_events
+        if (e == hipSuccess) e = gs_launch_zero(ctx->d_meta + q_base, 2 * nl, s0);    // queues, cycles
+        if (e == hipSuccess) e = hipEventRecord(ctx->fork, s0);
+        // model groups on up to four streams
+        std::vector<int> group_of(nl);
+        int n_groups = 0;
+        for (size_t l = 0; l < nl; ++l) {
+            if (l == 0 || launches[l].model != launches[l - 1].model) ++n_groups;
+            group_of[l] = n_groups - 1;
+        }
+        bool used[3] = {false, false, false};
+        for (size_t l = 0; l < nl && e == hipSuccess; ++l) {
+            const int g = group_of[l] % 4;
+            hipStream_t sg = g == 0 ? s0 : ctx->side[g - 1];
+            if (g > 0 && !used[g - 1]) {
+                e = hipStreamWaitEvent(sg, ctx->fork, 0);
+                used[g - 1] = true;
+                if (e != hipSuccess) break;
+            }
+            const Launch &L = launches[l];
+            DevOut O;
+            O.queue = ctx->d_meta + q_base + l;
+            O.cycles = ctx->d_meta + c_base + l;
+            O.g_count = W.count + (size_t)L.base * GS_MAX_BINS;
+            O.g_digits = W.digits + (size_t)L.base * gs::kDigitsPerSim;
+            O.g_stats = W.stats + (size_t)L.base * gs::ST_COUNT_;
+            O.events = ctx->d_events;
+            O.ev_off = ctx->d_meta + L.base;
+            O.ev_cap = ctx->d_meta + m + L.base;
+            O.n_events = ctx->d_meta + 2 * m + L.base;
+            e = gs_launch_traj(L.model, true, ctx->d_params + L.base, L.count, ctx->d_meta + L.off_pos, O,
+                               L.grid, sg);
+        }
+        for (int g = 0; g < 3 && e == hipSuccess; ++g) {
+            if (!used[g]) continue;
+            e = hipEventRecord(ctx->join[g], ctx->side[g]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(s0, ctx->join[g], 0);
+        }
+        if (e == hipSuccess) {
+            DevOut O;
+            O.queue = nullptr; O.cycles = nullptr;
+            O.g_count = W.count; O.g_digits = W.digits; O.g_stats = W.stats;
+            O.events = ctx->d_events;
+            O.ev_off = ctx->d_meta; O.ev_cap = ctx->d_meta + m; O.n_events = ctx->d_meta + 2 * m;
+            e = gs_launch_fold(ctx->d_params, (uint32_t)m, O, lds, fb, s0);
+        }
+        hipGraph_t graph = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(s0, &graph);
+        if (e == hipSuccess) e = e2;
+        if (e == hipSuccess) e = hipGraphInstantiate(&ctx->graph_exec, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            ctx->graph_exec = nullptr;
+            return fail(ctx, GS_ERR_HIP, "batch graph: %s", hipGetErrorString(e));
+        }
+        ctx->graph_key = key;
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    HIP_TRY(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    std::vector<uint64_t> host(m * words_per_sim());
+    std::vector<uint64_t> hmeta(meta_words);
+    HIP_TRY(ctx, hipMemcpyAsync(host.data(), ctx->d_words, host.size() * sizeof(uint64_t),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(hmeta.data(), ctx->d_meta, meta_words * sizeof(uint64_t),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    acc.kernel_ms += ms;
+    const Layout H = layout(host.data(), m);
+    for (size_t j = 0; j < m; ++j) {
+        const uint32_t i = order[j];
+        const uint64_t nev = hmeta[2 * m + j];
+        if (nev > hmeta[m + j]) { redo.push_back(i); continue; }
+        uint64_t *stats = H.stats + j * gs::ST_COUNT_;
+        stats[gs::ST_EVENTS] = nev;
+        fill_spectrum(sims[i], H.count + j * GS_MAX_BINS, H.digits + j * gs::kDigitsPerSim, stats, &outs[i]);
+        acc.active_cycles += stats[gs::ST_STEPS];
+    }
+    for (size_t l = 0; l < nl; ++l) {
+        acc.lane_cycles += hmeta[c_base + l] * (uint64_t)gs::kWave * gs::kCycleSteps;
+        acc.grid_blocks += launches[l].grid;
+    }
+    return GS_OK;
 }
 
 }  // namespace
@@ -281,7 +509,11 @@ gs_status gs_create(gs_ctx **out, int device) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
-    if (e == hipSuccess) e = hipMalloc((void **)&ctx->d_nev, sizeof(uint64_t));
+    for (int i = 0; i < 3 && e == hipSuccess; ++i) {
+        e = hipStreamCreateWithFlags(&ctx->side[i], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->join[i], hipEventDisableTiming);
+    }
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming);
     if (e != hipSuccess) {
         gs_destroy(ctx);
         return GS_ERR_HIP;
@@ -297,7 +529,13 @@ void gs_destroy(gs_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->d_words) (void)hipFree(ctx->d_words);
     if (ctx->d_events) (void)hipFree(ctx->d_events);
-    if (ctx->d_nev) (void)hipFree(ctx->d_nev);
+    if (ctx->d_meta) (void)hipFree(ctx->d_meta);
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    for (int i = 0; i < 3; ++i) {
+        if (ctx->side[i]) (void)hipStreamDestroy(ctx->side[i]);
+        if (ctx->join[i]) (void)hipEventDestroy(ctx->join[i]);
+    }
+    if (ctx->fork) (void)hipEventDestroy(ctx->fork);
     if (ctx->d_params) (void)hipFree(ctx->d_params);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -351,6 +589,36 @@ gs_status gs_set_launch(gs_ctx *ctx, int32_t model, uint32_t block_threads, uint
     return GS_OK;
 }
 
+gs_status gs_autotune(gs_ctx *ctx, const gs_params *pilot, uint32_t repeats, uint32_t *best_bpcu,
+                      double *best_steps_per_s) {
+    if (!ctx) return GS_ERR_INVALID_ARG;
+    std::string msg;
+    if (!check_params(pilot, msg)) return fail(ctx, GS_ERR_INVALID_ARG, "invalid pilot: %s", msg.c_str());
+    if (repeats == 0) repeats = 1;
+    const int model = pilot->model;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int max_bpcu = 0;
+    HIP_TRY(ctx, gs_traj_occupancy(model, false, &max_bpcu));
+    if (max_bpcu <= 0) max_bpcu = 1;
+    const uint32_t saved = ctx->bpcu_override[model];
+    gs_spectrum out;
+    double best = -1.0;
+    uint32_t best_b = 0;
+    for (int b = 1; b <= max_bpcu; ++b) {
+        ctx->bpcu_override[model] = (uint32_t)b;
+        for (uint32_t r = 0; r < repeats; ++r) {
+            const gs_status st = gs_run_plain(ctx, pilot, &out, nullptr, 0, nullptr);
+            if (st != GS_OK) { ctx->bpcu_override[model] = saved; return st; }
+            const double rate = ctx->stats.kernel_ms > 0 ? (double)out.total_steps / (ctx->stats.kernel_ms * 1e-3) : 0.0;
+            if (rate > best) { best = rate; best_b = (uint32_t)b; }
+        }
+    }
+    ctx->bpcu_override[model] = best_b;
+    if (best_bpcu) *best_bpcu = best_b;
+    if (best_steps_per_s) *best_steps_per_s = best;
+    return GS_OK;
+}
+
 gs_status gs_last_run_stats(const gs_ctx *ctx, gs_run_stats *st) {
     if (!ctx || !st) return GS_ERR_INVALID_ARG;
     *st = ctx->stats;
@@ -368,33 +636,44 @@ gs_status gs_run_plain(gs_ctx *ctx, const gs_params *p, gs_spectrum *out, gs_exi
     gs_status st = ensure_sims(ctx, 1);
     if (st == GS_OK) st = ensure_params(ctx, 1);
     if (st != GS_OK) return st;
+    if (st == GS_OK) st = ensure_meta(ctx, 64);
+    if (st != GS_OK) return st;
     const bool want_recs = recs != nullptr && rec_cap > 0;
     Layout L = layout(ctx->d_words, 1);
     HIP_TRY(ctx, gs_launch_zero(ctx->d_words, words_per_sim(), ctx->stream));
-    const unsigned grid = grid_for(ctx, p->model);
+    const unsigned grid = grid_for(ctx, p->model, false);
     std::vector<gs_exit_record> kept;
     uint64_t total_events = 0;
     double kernel_ms = 0.0;
     const uint64_t N = p->n_particles;
+    // d_meta: [0] event offset (0), [1] event capacity, [2] event count, [3] cycles
+    DevOut O;
+    O.queue = L.queue; O.g_count = L.count; O.g_digits = L.digits; O.g_stats = L.stats;
+    O.ev_off = ctx->d_meta; O.ev_cap = ctx->d_meta + 1; O.n_events = ctx->d_meta + 2;
+    O.cycles = ctx->d_meta + 3;
+    ctx->h_meta.assign(2, 0);
     for (uint64_t c0 = 0; c0 < N || (N == 0 && c0 == 0); c0 += kMaxChunk) {
         gs_params q = *p;
         q.first_particle = p->first_particle + c0;
         q.n_particles = std::min(kMaxChunk, N - c0);
-        const DevParams d = derive(q);
+        ctx->h_params.assign(1, derive(q));
         st = ensure_events(ctx, event_room(q.model, q.n_particles));
         if (st != GS_OK) return st;
-        HIP_TRY(ctx, hipMemcpyAsync(ctx->d_params, &d, sizeof d, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->d_params, ctx->h_params.data(), sizeof(DevParams),
+                                    hipMemcpyHostToDevice, ctx->stream));
         uint64_t nev = 0;
         for (int attempt = 0; attempt < 2; ++attempt) {
+            ctx->h_meta[0] = 0;
+            ctx->h_meta[1] = ctx->cap_events;
+            HIP_TRY(ctx, hipMemcpyAsync(ctx->d_meta, ctx->h_meta.data(), 2 * sizeof(uint64_t),
+                                        hipMemcpyHostToDevice, ctx->stream));
             HIP_TRY(ctx, gs_launch_zero(L.queue, 1, ctx->stream));
-            HIP_TRY(ctx, gs_launch_zero(ctx->d_nev, 1, ctx->stream));
-            DevOut O;
-            O.queue = L.queue; O.g_count = L.count; O.g_digits = L.digits; O.g_stats = L.stats;
-            O.events = ctx->d_events; O.ev_cap = ctx->cap_events; O.n_events = ctx->d_nev;
+            HIP_TRY(ctx, gs_launch_zero(ctx->d_meta + 2, 2, ctx->stream));
+            O.events = ctx->d_events;
             HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-            HIP_TRY(ctx, gs_launch_plain(d, ctx->d_params, O, grid, ctx->stream));
+            HIP_TRY(ctx, gs_launch_traj(q.model, false, ctx->d_params, 1, nullptr, O, grid, ctx->stream));
             HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-            HIP_TRY(ctx, hipMemcpyAsync(&nev, ctx->d_nev, sizeof nev, hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(ctx, hipMemcpyAsync(&nev, ctx->d_meta + 2, sizeof nev, hipMemcpyDeviceToHost, ctx->stream));
             HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
             float ms = 0.f;
             HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -403,15 +682,13 @@ gs_status gs_run_plain(gs_ctx *ctx, const gs_params *p, gs_spectrum *out, gs_exi
             // The buffer was too small (1D F-p crossings are unbounded): the
             // trajectories are deterministic, so run the chunk again with room
             // for every event.  Statistics of the first attempt are discarded.
+            if (attempt == 1) return fail(ctx, GS_ERR_OVERFLOW, "event buffer overflow");
             st = ensure_events(ctx, nev + nev / 8);
             if (st != GS_OK) return st;
             HIP_TRY(ctx, gs_launch_zero(L.stats, gs::ST_COUNT_, ctx->stream));
-            if (attempt == 1) return fail(ctx, GS_ERR_OVERFLOW, "event buffer overflow");
         }
-        DevOut O;
-        O.queue = L.queue; O.g_count = L.count; O.g_digits = L.digits; O.g_stats = L.stats;
-        O.events = ctx->d_events; O.ev_cap = ctx->cap_events; O.n_events = ctx->d_nev;
-        HIP_TRY(ctx, gs_launch_fold(d, ctx->d_params, O, nev, ctx->stream));
+        HIP_TRY(ctx, gs_launch_fold(ctx->d_params, 1, O, fold_lds_bytes(q.n_bins, q.n_weights),
+                                    fold_blocks(nev), ctx->stream));
         total_events += nev;
         if (want_recs && nev > 0 && kept.size() < rec_cap) {
             std::vector<gs::DevEvent> h(nev);
@@ -494,13 +771,71 @@ gs_status gs_host_math(int32_t fn, const double *x, double *y, uint64_t n) {
     return GS_OK;
 }
 
+gs_status gs_set_batch_budget(gs_ctx *ctx, uint64_t max_events) {
+    if (!ctx) return GS_ERR_INVALID_ARG;
+    if (max_events < (1u << 16)) return fail(ctx, GS_ERR_INVALID_ARG, "budget below 65536 events");
+    ctx->batch_event_budget = max_events;
+    return GS_OK;
+}
+
 gs_status gs_run_batch(gs_ctx *ctx, const gs_params *sims, uint32_t n_sims, gs_spectrum *outs) {
     if (!ctx) return GS_ERR_INVALID_ARG;
-    if (!sims || !outs) return fail(ctx, GS_ERR_INVALID_ARG, "NULL argument");
+    if ((!sims || !outs) && n_sims) return fail(ctx, GS_ERR_INVALID_ARG, "NULL argument");
     for (uint32_t i = 0; i < n_sims; ++i) {
-        gs_status st = gs_run_plain(ctx, &sims[i], &outs[i], nullptr, 0, nullptr);
-        if (st != GS_OK) return st;
+        std::string msg;
+        if (!check_params(&sims[i], msg))
+            return fail(ctx, GS_ERR_INVALID_ARG, "invalid params of simulation %u: %s", i, msg.c_str());
     }
+    const auto t_start = std::chrono::steady_clock::now();
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    gs_run_stats acc{};
+    acc.block_threads = gs::kBlockThreads;
+    // Simulations too large for one event region run on their own.
+    std::vector<uint32_t> small;
+    for (uint32_t i = 0; i < n_sims; ++i) {
+        const gs_params &p = sims[i];
+        if (p.n_particles > kMaxChunk || event_room(p.model, p.n_particles) > ctx->batch_event_budget / 2) {
+            gs_status st = gs_run_plain(ctx, &p, &outs[i], nullptr, 0, nullptr);
+            if (st != GS_OK) return st;
+            acc.kernel_ms += ctx->stats.kernel_ms;
+            acc.lane_cycles += ctx->stats.lane_cycles;
+            acc.active_cycles += ctx->stats.active_cycles;
+        } else {
+            small.push_back(i);
+        }
+    }
+    // Largest estimated work first, then cut into sub-batches that fit the
+    // event budget (one persistent launch group each).
+    std::stable_sort(small.begin(), small.end(), [&](uint32_t a, uint32_t b) {
+        return est_cost(sims[a]) * (double)sims[a].n_particles > est_cost(sims[b]) * (double)sims[b].n_particles;
+    });
+    size_t i0 = 0;
+    while (i0 < small.size()) {
+        uint64_t room = 0;
+        size_t i1 = i0;
+        while (i1 < small.size() && i1 - i0 < kMaxBatchSims) {
+            const uint64_t r = event_room(sims[small[i1]].model, sims[small[i1]].n_particles);
+            if (i1 > i0 && room + r > ctx->batch_event_budget) break;
+            room += r;
+            ++i1;
+        }
+        std::vector<uint32_t> part(small.begin() + i0, small.begin() + i1);
+        std::vector<uint32_t> redo;
+        gs_status st = run_sub_batch(ctx, sims, part, outs, redo, acc);
+        if (st != GS_OK) return st;
+        for (uint32_t i : redo) {   // event region overflow (1D F-p): run alone
+            st = gs_run_plain(ctx, &sims[i], &outs[i], nullptr, 0, nullptr);
+            if (st != GS_OK) return st;
+            acc.kernel_ms += ctx->stats.kernel_ms;
+            acc.lane_cycles += ctx->stats.lane_cycles;
+            acc.active_cycles += ctx->stats.active_cycles;
+        }
+        i0 = i1;
+    }
+    acc.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    ctx->stats = acc;
+    for (uint32_t i = 0; i < n_sims; ++i)
+        if (outs[i].n_acc_overflow) return fail(ctx, GS_ERR_OVERFLOW, "weight accumulator overflow (simulation %u)", i);
     return GS_OK;
 }
 

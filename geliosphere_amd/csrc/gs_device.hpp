@@ -262,14 +262,14 @@ enum { STEP_CONTINUE = 0, STEP_EVENT = 1, STEP_EVENT_DONE = 2, STEP_DONE = 3 };
 struct Bp {
     static constexpr int kNormals = 1;
     double r, P, beta, t_inj;
-    __device__ __forceinline__ bool init(const DevParams &Pm, double T) {
+    __device__ __forceinline__ bool init(const StepParams &Pm, double T) {
         t_inj = T;
         P = fm::sqrt_pos(T * (T + 2.0 * kT0));
         beta = P * fm::rcp(T + kT0);
         r = Pm.r_inj;
         return r < 100.0002;
     }
-    __device__ __forceinline__ int step(const DevParams &Pm, const double *z, Event &ev) {
+    __device__ __forceinline__ int step(const StepParams &Pm, const double *z, Event &ev) {
         const double K2dt = Pm.two_dt * (Pm.K0 * beta * P);          // 2 K dt
         const double inv_r = fm::rcp(r);
         const double Pn = fma(-(Pm.cp * P), inv_r, P);               // p - 2 V p dt / (3 r)
@@ -305,7 +305,7 @@ struct Bp {
 struct Fp {
     static constexpr int kNormals = 1;
     double r, P, beta, R, sumac, t_inj;
-    __device__ __forceinline__ bool init(const DevParams &Pm, double T) {
+    __device__ __forceinline__ bool init(const StepParams &Pm, double T) {
         t_inj = T;
         P = fm::sqrt_pos(T * (T + 2.0 * kT0));
         R = P;
@@ -314,7 +314,7 @@ struct Fp {
         r = 100.0001;
         return true;
     }
-    __device__ __forceinline__ int step(const DevParams &Pm, const double *z, Event &ev) {
+    __device__ __forceinline__ int step(const StepParams &Pm, const double *z, Event &ev) {
         const double K2dt = Pm.two_dt * (Pm.K0 * beta * R);
         const double inv_r = fm::rcp(r);
         const double dr = fma(K2dt, inv_r, Pm.vdt) + z[0] * fm::sqrt_pos(K2dt);
@@ -350,14 +350,14 @@ struct Fp {
 struct Sp {
     static constexpr int kNormals = 2;
     double r, theta, T, t_inj;
-    __device__ __forceinline__ bool init(const DevParams &Pm, double Tin) {
+    __device__ __forceinline__ bool init(const StepParams &Pm, double Tin) {
         t_inj = Tin;
         T = Tin;
         r = Pm.r_inj;
         theta = Pm.theta_inj;
         return r < 100.0;
     }
-    __device__ __forceinline__ int step(const DevParams &Pm, const double *z, Event &ev) {
+    __device__ __forceinline__ int step(const StepParams &Pm, const double *z, Event &ev) {
         double s, c;
         sincos(theta, &s, &c);
         const double Rig = fm::sqrt_pos(T * (T + 2.0 * kT0));
@@ -427,14 +427,14 @@ struct Sp {
 struct Ge {
     static constexpr int kNormals = 3;
     double r, theta, T, t_inj;
-    __device__ __forceinline__ bool init(const DevParams &Pm, double Tin) {
+    __device__ __forceinline__ bool init(const StepParams &Pm, double Tin) {
         t_inj = Tin;
         T = Tin;
         r = Pm.r_inj;
         theta = Pm.theta_inj;
         return r < 100.0;
     }
-    __device__ __forceinline__ int step(const DevParams &Pm, const double *z, Event &ev) {
+    __device__ __forceinline__ int step(const StepParams &Pm, const double *z, Event &ev) {
         constexpr double kf = 1.0 / (3.0 * (5.0 / 3.4));   // as written in GeliosphereCpuModel.cpp:210-299
         double s, c;
         sincos(theta, &s, &c);

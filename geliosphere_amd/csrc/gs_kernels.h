@@ -12,6 +12,7 @@ constexpr int kWave = 64;
 constexpr int kBlockThreads = 256;
 constexpr int kCycleSteps = 4;     // steps between refills (whole Philox blocks for every model)
 constexpr uint64_t kChunk = 64;    // particle ids taken from the global queue per grab
+constexpr int kBatchMaxSims = 1024; // simulations per batch trajectory launch (offsets in LDS)
 
 // Indices of the statistics words (LDS and global).
 enum Stat {
@@ -28,17 +29,11 @@ enum Stat {
     ST_COUNT_
 };
 
-// Parameters of one simulation as the kernels read them (host-derived
-// constants included).  Passed by value (plain runs) or from an array in
-// global memory (batched runs).
-struct DevParams {
-    int32_t model, injection;
-    uint32_t flags, n_bins, n_weights, grid_n;
-    int32_t weights[GS_MAX_WEIGHTS];
-    double K0, V, dt, r_inj, theta_inj, tilt, polarity, ratio, delta0, T_min, T_max;
-    // derived
-    double log_ratio;     // ln(T_max / T_min)
-    double grid_step;     // (T_max - T_min) / grid_n
+// Parameters a model's init/step read (host-derived constants included).
+// Batched runs copy them per lane into registers; plain runs read them
+// wave-uniformly.
+struct StepParams {
+    double K0, V, dt, r_inj, theta_inj, tilt, polarity, ratio, delta0;
     double sqdt;          // sqrt(dt)
     double two_dt;        // 2 dt
     double vdt;           // signed V dt as the model uses it (1D B-p: -V dt)
@@ -48,10 +43,19 @@ struct DevParams {
     double omega2_v2;     // omega^2 / V^2
     double th_lo, th_hi;  // 1.7 deg and 178.3 deg in rad, as the reference writes them
     double pol_theta;     // polarity used in the 2D theta drift
+};
+
+// Parameters of one simulation.
+struct DevParams : StepParams {
+    int32_t model, injection;
+    uint32_t flags, n_bins, n_weights, grid_n;
+    int32_t weights[GS_MAX_WEIGHTS];
+    double T_min, T_max;
+    double log_ratio;     // ln(T_max / T_min)
+    double grid_step;     // (T_max - T_min) / grid_n
     uint64_t n_particles, first_particle, seed, max_steps;
     double edges[GS_MAX_BINS + 1];
 };
-
 
 // One event as the trajectory kernels store it (exit or 1 AU crossing).
 struct DevEvent {
@@ -61,17 +65,20 @@ struct DevEvent {
     uint32_t pad;
 };
 
-// Global outputs of a launch.  Per simulation: count[GS_MAX_BINS],
-// digits[GS_MAX_WEIGHTS][2][GS_MAX_BINS][GS_ACC_LIMBS] (32-bit values in
-// 64-bit words), stats[ST_COUNT_].
+// Device buffers of a launch over n_sims simulations (plain runs: 1).
+// Per simulation s: histogram count[GS_MAX_BINS], digits[GS_MAX_WEIGHTS][2]
+// [GS_MAX_BINS][GS_ACC_LIMBS] (32-bit values in 64-bit words), stats
+// [ST_COUNT_], and an event region events[ev_off[s] .. ev_off[s]+ev_cap[s]).
 struct DevOut {
-    uint64_t *queue;      // work queue (particle ids handed out)
+    uint64_t *queue;          // virtual particle ids handed out
     uint64_t *g_count;
     uint64_t *g_digits;
     uint64_t *g_stats;
-    DevEvent *events;     // event buffer
-    uint64_t ev_cap;
-    uint64_t *n_events;   // events produced (may exceed ev_cap: then rerun with more room)
+    DevEvent *events;
+    const uint64_t *ev_off;
+    const uint64_t *ev_cap;
+    uint64_t *n_events;       // per simulation; may exceed ev_cap (then re-run with more room)
+    uint64_t *cycles;         // wave cycles issued by the launch
 };
 
 // LDS layout per workgroup: count[n_bins] | digits[n_weights][2][n_bins][16] | stat[ST_COUNT_]
@@ -85,10 +92,15 @@ constexpr size_t kDigitsPerSim = (size_t)GS_MAX_WEIGHTS * 2 * GS_MAX_BINS * GS_A
 hipError_t gs_launch_zero(uint64_t *p, size_t n, hipStream_t s);
 hipError_t gs_launch_debug(int fn, const double *x, double *y, uint64_t n, uint64_t seed,
                            uint64_t pid, hipStream_t s);
-hipError_t gs_plain_occupancy(int model, int *blocks_per_cu);
-// Trajectories of one simulation.  P: host copy, dP: the same block in device memory.
-hipError_t gs_launch_plain(const gs::DevParams &P, const gs::DevParams *dP, const gs::DevOut &O,
-                           unsigned grid, hipStream_t s);
-// Histogram of the events [0, n) of one simulation.
-hipError_t gs_launch_fold(const gs::DevParams &P, const gs::DevParams *dP, const gs::DevOut &O,
-                          uint64_t n, hipStream_t s);
+// Blocks of `kBlockThreads` a CU holds for the trajectory kernel of `model`.
+hipError_t gs_traj_occupancy(int model, bool batch, int *blocks_per_cu);
+// Trajectories of n_sims simulations of one model.  sims: device array;
+// offsets: device array of n_sims + 1 prefix sums of n_particles (the
+// virtual id space).  batch = false requires n_sims == 1 (uniform parameters).
+hipError_t gs_launch_traj(int model, bool batch, const gs::DevParams *sims, uint32_t n_sims,
+                          const uint64_t *offsets, const gs::DevOut &O, unsigned grid,
+                          hipStream_t s);
+// Histograms of n_sims simulations from their event regions; lds_bytes fits
+// the largest (n_bins, n_weights) among them.
+hipError_t gs_launch_fold(const gs::DevParams *sims, uint32_t n_sims, const gs::DevOut &O,
+                          size_t lds_bytes, unsigned blocks_per_sim, hipStream_t s);

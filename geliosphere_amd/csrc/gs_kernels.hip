@@ -16,26 +16,58 @@
 
 namespace gs {
 
-// Append one event to the event buffer (counted even when the buffer is full).
+// Append one event to its simulation's event region (counted even when the
+// region is full; the host then runs the simulation again with more room).
 __device__ __forceinline__ void emit(const DevOut &O, const Event &ev, uint64_t pid, uint32_t sim) {
-    const unsigned long long i = atomicAdd((unsigned long long *)O.n_events, 1ull);
-    if (i < O.ev_cap) {
+    const unsigned long long i = atomicAdd((unsigned long long *)&O.n_events[sim], 1ull);
+    if (i < O.ev_cap[sim]) {
         DevEvent d;
         d.t_inj = ev.t_inj; d.t_exit = ev.t_exit; d.r = ev.r; d.theta = ev.theta;
         d.sumac = ev.sumac; d.id = pid; d.sim = sim; d.pad = 0;
-        O.events[i] = d;
+        O.events[O.ev_off[sim] + i] = d;
     }
 }
 
-template <class Model>
-__device__ void run_lanes(const DevParams &P, uint64_t *stat, unsigned long long *queue,
-                          const DevOut &O, uint32_t sim) {
+// Per-simulation statistics a batch lane has gathered since it moved to its
+// current simulation.
+__device__ __forceinline__ void flush_sim_stats(const DevOut &O, uint32_t sim, uint64_t steps,
+                                                uint32_t particles, uint32_t nan, uint32_t capped) {
+    unsigned long long *g = (unsigned long long *)O.g_stats + (size_t)sim * ST_COUNT_;
+    if (steps) atomicAdd(&g[ST_STEPS], (unsigned long long)steps);
+    if (particles) atomicAdd(&g[ST_PARTICLES], (unsigned long long)particles);
+    if (nan) atomicAdd(&g[ST_NAN], (unsigned long long)nan);
+    if (capped) atomicAdd(&g[ST_CAP], (unsigned long long)capped);
+}
+
+// The lanes of one wave integrate particles until the id space is exhausted.
+//
+// kBatch = false: one simulation, parameters read wave-uniformly from sims[0];
+//   per-workgroup statistics go to `stat` (LDS).
+// kBatch = true: n_sims simulations of one model concatenated into a virtual
+//   id space (offsets `off`, in LDS: simulation s owns virtual ids
+//   [off[s], off[s+1])).  A lane that takes a virtual id outside its current
+//   simulation's range looks the simulation up, flushes its statistics for
+//   the old one and loads the new one's step parameters, Philox key and step
+//   cap into registers.  A wave only ever moves forward in the id space, so a
+//   lane's simulation index never decreases.
+template <class Model, bool kBatch>
+__device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, const uint64_t *off,
+                          uint64_t *stat, const DevOut &O) {
     constexpr int NPS = Model::kNormals;   // normals per step; a cycle uses NPS Philox blocks
+    unsigned long long *queue = (unsigned long long *)O.queue;
     const int lane = (int)(threadIdx.x & (kWave - 1));
     const uint64_t lane_lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const uint32_t k0 = (uint32_t)P.seed, k1 = (uint32_t)(P.seed >> 32);
-    const uint64_t N = P.n_particles;
-    const uint32_t cap = (uint32_t)P.max_steps;
+    const DevParams &P0 = sims[0];
+    const uint64_t N = kBatch ? off[n_sims] : P0.n_particles;
+
+    // Per-simulation values (wave-uniform for plain runs, per lane in batches).
+    uint32_t k0 = (uint32_t)P0.seed, k1 = (uint32_t)(P0.seed >> 32);
+    uint32_t cap = (uint32_t)P0.max_steps;
+    uint64_t first = P0.first_particle;
+    uint32_t sim = 0;
+    uint64_t sim_lo = 0, sim_hi = 0;
+    StepParams lp;
+    if constexpr (kBatch) lp = StepParams{};
 
     Model m;
     uint64_t pid = 0;
@@ -45,6 +77,11 @@ __device__ void run_lanes(const DevParams &P, uint64_t *stat, unsigned long long
     bool exhausted = false;
     uint64_t my_steps = 0;
     uint32_t my_particles = 0, my_nan = 0, my_cap = 0, cycles = 0;
+
+    auto do_step = [&](const double *z, Event &ev) -> int {
+        if constexpr (kBatch) return m.step(lp, z, ev);
+        else return m.step(P0, z, ev);
+    };
 
     for (;;) {
         // ---- refill idle lanes (wave-uniform control flow)
@@ -77,11 +114,41 @@ __device__ void run_lanes(const DevParams &P, uint64_t *stat, unsigned long long
             }
             if (take != ~0ull) {
                 my_steps += steps;
-                pid = P.first_particle + take;
                 steps = 0;
                 blk = 0;
-                ++my_particles;
-                active = m.init(P, injection_energy(P, pid));
+                double T;
+                if constexpr (kBatch) {
+                    if (take >= sim_hi) {
+                        if (my_particles) flush_sim_stats(O, sim, my_steps, my_particles, my_nan, my_cap);
+                        my_steps = 0;
+                        my_particles = my_nan = my_cap = 0;
+                        // largest s in [sim, n_sims) with off[s] <= take; off[s+1] > take follows
+                        uint32_t lo = sim, hi = n_sims;
+                        while (hi - lo > 1) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            if (off[mid] <= take) lo = mid; else hi = mid;
+                        }
+                        sim = lo;
+                        sim_lo = off[lo];
+                        sim_hi = off[lo + 1];
+                        const DevParams &Q = sims[sim];
+                        lp = static_cast<const StepParams &>(Q);
+                        k0 = (uint32_t)Q.seed;
+                        k1 = (uint32_t)(Q.seed >> 32);
+                        cap = (uint32_t)Q.max_steps;
+                        first = Q.first_particle;
+                    }
+                    pid = first + (take - sim_lo);
+                    const DevParams &Q = sims[sim];
+                    T = injection_energy(Q, pid);
+                    ++my_particles;
+                    active = m.init(lp, T);
+                } else {
+                    pid = first + take;
+                    T = injection_energy(P0, pid);
+                    ++my_particles;
+                    active = m.init(P0, T);
+                }
             }
         }
         if (__ballot(active) == 0) {
@@ -103,7 +170,7 @@ __device__ void run_lanes(const DevParams &P, uint64_t *stat, unsigned long long
                         active = false;
                     } else {
                         Event ev;
-                        const int rc = m.step(P, &z[s], ev);
+                        const int rc = do_step(&z[s], ev);
                         ++steps;
                         if (rc == STEP_EVENT || rc == STEP_EVENT_DONE) emit(O, ev, pid, sim);
                         if (rc >= STEP_EVENT_DONE) {
@@ -142,7 +209,7 @@ __device__ void run_lanes(const DevParams &P, uint64_t *stat, unsigned long long
                         active = false;
                     } else {
                         Event ev;
-                        const int rc = m.step(P, z, ev);
+                        const int rc = do_step(z, ev);
                         ++steps;
                         if (rc == STEP_EVENT || rc == STEP_EVENT_DONE) emit(O, ev, pid, sim);
                         if (rc >= STEP_EVENT_DONE) {
@@ -156,11 +223,16 @@ __device__ void run_lanes(const DevParams &P, uint64_t *stat, unsigned long long
         blk += (uint32_t)NPS;
     }
     my_steps += steps;
-    atomicAdd((unsigned long long *)&stat[ST_STEPS], (unsigned long long)my_steps);
-    atomicAdd((unsigned long long *)&stat[ST_PARTICLES], (unsigned long long)my_particles);
-    if (my_nan) atomicAdd((unsigned long long *)&stat[ST_NAN], (unsigned long long)my_nan);
-    if (my_cap) atomicAdd((unsigned long long *)&stat[ST_CAP], (unsigned long long)my_cap);
-    if (lane == 0) atomicAdd((unsigned long long *)&stat[ST_CYCLES], (unsigned long long)cycles);
+    if constexpr (kBatch) {
+        if (my_particles) flush_sim_stats(O, sim, my_steps, my_particles, my_nan, my_cap);
+        if (lane == 0) atomicAdd((unsigned long long *)O.cycles, (unsigned long long)cycles);
+    } else {
+        atomicAdd((unsigned long long *)&stat[ST_STEPS], (unsigned long long)my_steps);
+        atomicAdd((unsigned long long *)&stat[ST_PARTICLES], (unsigned long long)my_particles);
+        if (my_nan) atomicAdd((unsigned long long *)&stat[ST_NAN], (unsigned long long)my_nan);
+        if (my_cap) atomicAdd((unsigned long long *)&stat[ST_CAP], (unsigned long long)my_cap);
+        if (lane == 0) atomicAdd((unsigned long long *)&stat[ST_CYCLES], (unsigned long long)cycles);
+    }
 }
 
 // Flush the workgroup histogram into the global one.  Digits are carry-
@@ -201,29 +273,48 @@ template <class Model>
 __global__ void __launch_bounds__(kBlockThreads) plain_kernel(const DevParams *__restrict__ Pp,
                                                               const DevOut O) {
     __shared__ uint64_t stat[ST_COUNT_];
-    const DevParams &P = *Pp;
     if (threadIdx.x < ST_COUNT_) stat[threadIdx.x] = 0;
     __syncthreads();
-    run_lanes<Model>(P, stat, (unsigned long long *)O.queue, O, 0);
+    run_lanes<Model, false>(Pp, 1, nullptr, stat, O);
     __syncthreads();
     if (threadIdx.x < ST_COUNT_ && stat[threadIdx.x])
         atomicAdd((unsigned long long *)&O.g_stats[threadIdx.x], (unsigned long long)stat[threadIdx.x]);
 }
 
-// Histogram pass: every workgroup folds a strided share of the events into its
-// LDS histogram and flushes it.  Sums are exact, so the order of events in the
-// buffer does not matter.
-__global__ void __launch_bounds__(kBlockThreads) fold_kernel(const DevParams *__restrict__ Pp,
-                                                             const DevOut O, uint64_t n) {
+// Batch of n_sims simulations of one model (n_sims <= kBatchMaxSims).
+// Lanes hold their simulation's step parameters in VGPRs; kMinWaves asks the
+// register allocator for that many waves per SIMD (2D models: 2, trading a
+// few spills for occupancy).
+template <class Model, int kMinWaves>
+__global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(kMinWaves))) batch_kernel(const DevParams *__restrict__ sims,
+                                                              uint32_t n_sims,
+                                                              const uint64_t *__restrict__ offsets,
+                                                              const DevOut O) {
+    __shared__ uint64_t off[kBatchMaxSims + 1];
+    for (uint32_t i = threadIdx.x; i <= n_sims; i += blockDim.x) off[i] = offsets[i];
+    __syncthreads();
+    run_lanes<Model, true>(sims, n_sims, off, nullptr, O);
+}
+
+// Histogram pass: workgroup (x, s) folds a strided share of simulation s's
+// events into its LDS histogram and flushes it.  Sums are exact, so the order
+// of events in a region does not matter.
+__global__ void __launch_bounds__(kBlockThreads) fold_kernel(const DevParams *__restrict__ sims,
+                                                             const DevOut O) {
     extern __shared__ uint64_t lds[];
-    const DevParams &P = *Pp;
+    const uint32_t sim = blockIdx.y;
+    const DevParams &P = sims[sim];
+    uint64_t n = O.n_events[sim];
+    const uint64_t cap = O.ev_cap[sim];
+    if (n > cap) n = cap;
+    const DevEvent *ev = O.events + O.ev_off[sim];
     zero_lds(lds, lds_words((int)P.n_bins, (int)P.n_weights));
     __syncthreads();
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x)
-        fold_one(P, lds, O.events[i]);
+        fold_one(P, lds, ev[i]);
     __syncthreads();
-    flush_hist(P, lds, O, 0);
+    flush_hist(P, lds, O, sim);
 }
 
 // Diagnostics: evaluate one math-layer function elementwise, or dump the
@@ -264,12 +355,18 @@ __global__ void zero_kernel(uint64_t *p, size_t n) {
 // ---------------------------------------------------------------- launchers
 using namespace gs;
 
-static const void *plain_kernel_ptr(int model) {
+#ifndef GS_BATCH_WAVES_GE
+#define GS_BATCH_WAVES_GE 2
+#endif
+constexpr int kBatchWaves1D = 1, kBatchWavesSp = 2, kBatchWavesGe = GS_BATCH_WAVES_GE;
+
+static const void *traj_kernel_ptr(int model, bool batch) {
     switch (model) {
-    case GS_MODEL_1D_FP: return (const void *)&plain_kernel<Fp>;
-    case GS_MODEL_1D_BP: return (const void *)&plain_kernel<Bp>;
-    case GS_MODEL_2D_SOLARPROP: return (const void *)&plain_kernel<Sp>;
-    default: return (const void *)&plain_kernel<Ge>;
+    case GS_MODEL_1D_FP: return batch ? (const void *)&batch_kernel<Fp, kBatchWaves1D> : (const void *)&plain_kernel<Fp>;
+    case GS_MODEL_1D_BP: return batch ? (const void *)&batch_kernel<Bp, kBatchWaves1D> : (const void *)&plain_kernel<Bp>;
+    case GS_MODEL_2D_SOLARPROP:
+        return batch ? (const void *)&batch_kernel<Sp, kBatchWavesSp> : (const void *)&plain_kernel<Sp>;
+    default: return batch ? (const void *)&batch_kernel<Ge, kBatchWavesGe> : (const void *)&plain_kernel<Ge>;
     }
 }
 
@@ -288,37 +385,41 @@ hipError_t gs_launch_zero(uint64_t *p, size_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t gs_plain_occupancy(int model, int *blocks_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, plain_kernel_ptr(model),
+hipError_t gs_traj_occupancy(int model, bool batch, int *blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, traj_kernel_ptr(model, batch),
                                                         kBlockThreads, 0);
 }
 
-hipError_t gs_launch_fold(const DevParams &P, const DevParams *dP, const DevOut &O, uint64_t n,
-                          hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    const size_t lds = (size_t)lds_words((int)P.n_bins, (int)P.n_weights) * sizeof(uint64_t);
-    uint64_t blocks = (n + kBlockThreads - 1) / kBlockThreads;
-    if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(fold_kernel, dim3((unsigned)blocks), dim3(kBlockThreads), lds, s, dP, O, n);
+template <class Model, int kMinWaves>
+static void launch_traj_t(bool batch, const DevParams *sims, uint32_t n_sims, const uint64_t *offsets,
+                          const DevOut &O, unsigned grid, hipStream_t s) {
+    if (batch)
+        hipLaunchKernelGGL((batch_kernel<Model, kMinWaves>), dim3(grid), dim3(kBlockThreads), 0, s, sims, n_sims,
+                           offsets, O);
+    else
+        hipLaunchKernelGGL(plain_kernel<Model>, dim3(grid), dim3(kBlockThreads), 0, s, sims, O);
+}
+
+hipError_t gs_launch_traj(int model, bool batch, const DevParams *sims, uint32_t n_sims,
+                          const uint64_t *offsets, const DevOut &O, unsigned grid, hipStream_t s) {
+    if (n_sims == 0 || grid == 0) return hipSuccess;
+    if ((!batch && n_sims != 1) || (batch && (n_sims > (uint32_t)kBatchMaxSims || !offsets)))
+        return hipErrorInvalidValue;
+    switch (model) {
+    case GS_MODEL_1D_FP: launch_traj_t<Fp, kBatchWaves1D>(batch, sims, n_sims, offsets, O, grid, s); break;
+    case GS_MODEL_1D_BP: launch_traj_t<Bp, kBatchWaves1D>(batch, sims, n_sims, offsets, O, grid, s); break;
+    case GS_MODEL_2D_SOLARPROP: launch_traj_t<Sp, kBatchWavesSp>(batch, sims, n_sims, offsets, O, grid, s); break;
+    case GS_MODEL_2D_GELIOSPHERE: launch_traj_t<Ge, kBatchWavesGe>(batch, sims, n_sims, offsets, O, grid, s); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
-hipError_t gs_launch_plain(const DevParams &P, const DevParams *dP, const DevOut &O, unsigned grid,
-                           hipStream_t s) {
-    const size_t lds = 0;
-    switch (P.model) {
-    case GS_MODEL_1D_FP:
-        hipLaunchKernelGGL(plain_kernel<Fp>, dim3(grid), dim3(kBlockThreads), lds, s, dP, O);
-        break;
-    case GS_MODEL_1D_BP:
-        hipLaunchKernelGGL(plain_kernel<Bp>, dim3(grid), dim3(kBlockThreads), lds, s, dP, O);
-        break;
-    case GS_MODEL_2D_SOLARPROP:
-        hipLaunchKernelGGL(plain_kernel<Sp>, dim3(grid), dim3(kBlockThreads), lds, s, dP, O);
-        break;
-    default:
-        hipLaunchKernelGGL(plain_kernel<Ge>, dim3(grid), dim3(kBlockThreads), lds, s, dP, O);
-        break;
-    }
+hipError_t gs_launch_fold(const DevParams *sims, uint32_t n_sims, const DevOut &O, size_t lds_bytes,
+                          unsigned blocks_per_sim, hipStream_t s) {
+    if (n_sims == 0 || blocks_per_sim == 0) return hipSuccess;
+    if (n_sims > 65535u) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fold_kernel, dim3(blocks_per_sim, n_sims), dim3(kBlockThreads), lds_bytes, s,
+                       sims, O);
     return hipGetLastError();
 }

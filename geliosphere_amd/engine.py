@@ -14,8 +14,9 @@ LIB_PATH = os.path.join(_HERE, "libgeliosphere_amd.so")
 _lib = None
 
 EXPORTS = ("gs_abi_version", "gs_create", "gs_destroy", "gs_last_error", "gs_default_params",
-           "gs_validate_params", "gs_run_plain", "gs_run_batch", "gs_set_launch",
-           "gs_last_run_stats", "gs_debug_math", "gs_debug_normals", "gs_host_math")
+           "gs_validate_params", "gs_run_plain", "gs_run_batch", "gs_set_batch_budget",
+           "gs_set_launch", "gs_autotune", "gs_last_run_stats", "gs_debug_math", "gs_debug_normals",
+           "gs_host_math")
 
 DBG_RCP, DBG_RSQRT, DBG_SQRT, DBG_LOG, DBG_EXP, DBG_SIN_TURN, DBG_COS_TURN, DBG_LOG_U32, \
     DBG_RAW_RCP, DBG_RAW_RSQ, DBG_RAW_SQRT = range(11)
@@ -52,8 +53,13 @@ def lib():
         L.gs_run_plain.restype = ctypes.c_int
         L.gs_run_batch.argtypes = [vp, P(abi.GsParams), ctypes.c_uint32, P(abi.GsSpectrum)]
         L.gs_run_batch.restype = ctypes.c_int
+        L.gs_set_batch_budget.argtypes = [vp, ctypes.c_uint64]
+        L.gs_set_batch_budget.restype = ctypes.c_int
         L.gs_set_launch.argtypes = [vp, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32]
         L.gs_set_launch.restype = ctypes.c_int
+        L.gs_autotune.argtypes = [vp, P(abi.GsParams), ctypes.c_uint32, P(ctypes.c_uint32),
+                                  P(ctypes.c_double)]
+        L.gs_autotune.restype = ctypes.c_int
         L.gs_last_run_stats.argtypes = [vp, P(abi.GsRunStats)]
         L.gs_last_run_stats.restype = ctypes.c_int
         dp = P(ctypes.c_double)
@@ -149,8 +155,27 @@ class Engine:
         self._check(rc, "gs_run_batch")
         return list(outs)
 
+    def run_batch_status(self, sims):
+        """Status code of a batch (no raise), for error-path tests."""
+        n = len(sims)
+        arr = (abi.GsParams * n)(*sims)
+        outs = (abi.GsSpectrum * max(n, 1))()
+        return lib().gs_run_batch(self._ctx, arr, n, outs)
+
+    def set_batch_budget(self, max_events):
+        self._check(lib().gs_set_batch_budget(self._ctx, max_events), "gs_set_batch_budget")
+
     def set_launch(self, model, block_threads=0, blocks_per_cu=0):
         self._check(lib().gs_set_launch(self._ctx, model, block_threads, blocks_per_cu), "gs_set_launch")
+
+    def autotune(self, pilot, repeats=1):
+        """Time the pilot at every blocks-per-CU setting; apply and return
+        (best blocks per CU, its particle-steps/s)."""
+        b = ctypes.c_uint32(0)
+        r = ctypes.c_double(0.0)
+        self._check(lib().gs_autotune(self._ctx, ctypes.byref(pilot), repeats, ctypes.byref(b),
+                                      ctypes.byref(r)), "gs_autotune")
+        return b.value, r.value
 
     def debug_math(self, fn, xs):
         n = len(xs)

@@ -194,14 +194,29 @@ gs_status gs_run_plain(gs_ctx *ctx, const gs_params *p, gs_spectrum *out,
                        gs_exit_record *recs, uint64_t rec_cap,
                        uint64_t *n_recs);
 
-/* Run n_sims independent simulations in one persistent launch.  outs[i] is
- * identical, bit for bit, to gs_run_plain(sims[i]). */
+/* Run n_sims independent simulations (any mix of models) together: one
+ * persistent launch per model over the concatenated particles of its
+ * simulations, the model groups concurrently, then one histogram pass; the
+ * launch sequence is a HIP graph reused while the batch layout stays the
+ * same.  outs[i] is identical, bit for bit, to gs_run_plain(sims[i]).
+ * Simulations whose event region would exceed half the batch event budget
+ * run alone.  gs_last_run_stats covers the whole batch. */
 gs_status gs_run_batch(gs_ctx *ctx, const gs_params *sims, uint32_t n_sims,
                        gs_spectrum *outs);
+/* Event records (56 B each) a batch may hold on the device at once
+ * (default 2^29); larger batches run as consecutive sub-batches. */
+gs_status gs_set_batch_budget(gs_ctx *ctx, uint64_t max_events);
 
 /* Override the launch shape used for `model` (0 = automatic). */
 gs_status gs_set_launch(gs_ctx *ctx, int32_t model, uint32_t block_threads,
                         uint32_t blocks_per_cu);
+/* Pick blocks per CU for `model` by timing the pilot simulation at every
+ * occupancy from 1 to the kernel's maximum (best of `repeats` runs each) and
+ * apply it as gs_set_launch would.  best_bpcu / best_steps_per_s may be NULL.
+ * Replaces the fixed 256-thread launch geometry of setThreadBlockSize
+ * (reference CUDAKernel/src/OneDimensionBpGpuModel.cpp:45-92). */
+gs_status gs_autotune(gs_ctx *ctx, const gs_params *pilot, uint32_t repeats,
+                      uint32_t *best_bpcu, double *best_steps_per_s);
 /* Statistics of the last run on ctx. */
 gs_status gs_last_run_stats(const gs_ctx *ctx, gs_run_stats *st);
 
