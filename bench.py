@@ -42,15 +42,23 @@ def parse():
 
 
 def calibrate_copy(mib):
-    """A plain copy of known size, to scale FETCH_SIZE / WRITE_SIZE readings."""
-    import torch
-    n = int(mib * (1 << 20)) // 4
-    x = torch.ones(n, dtype=torch.float32, device="cuda")
-    y = torch.empty_like(x)
-    torch.cuda.synchronize()
-    y.copy_(x)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
+    """One device-to-device hipMemcpy (the runtime's copy kernel) of known
+    size, to scale FETCH_SIZE / WRITE_SIZE readings into bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = int(mib * (1 << 20))
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for ptr in (src, dst):
+        if hip.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(n)):
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, ctypes.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    if hip.hipMemcpy(dst, src, ctypes.c_size_t(n), 3):   # hipMemcpyDeviceToDevice
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n}), flush=True)
 
 
 def cpu_baseline(budget_s):
