@@ -265,7 +265,7 @@ unsigned fold_blocks(uint64_t events) {
 }
 
 // Particles per trajectory launch: bounds the event buffer (B-p and 2D models
-// produce at most one event per particle) to 2^27 x 56 B = 7.5 GB.
+// produce at most one event per particle) to 2^27 x 64 B = 8.6 GB.
 const uint64_t kMaxChunk = 1ull << 27;
 
 uint64_t event_room(int model, uint64_t n) {

@@ -58,12 +58,16 @@ struct DevParams : StepParams {
 };
 
 // One event as the trajectory kernels store it (exit or 1 AU crossing).
-struct DevEvent {
+// One whole 64-byte line per record: aligned full-line stores need no
+// read-modify-write in L2.
+struct alignas(64) DevEvent {
     double t_inj, t_exit, r, theta, sumac;
     uint64_t id;
-    uint32_t sim;   // simulation index within a batch (0 for plain runs)
+    uint32_t sim;   // simulation index within its launch (0 for plain runs)
     uint32_t pad;
+    uint64_t pad2;
 };
+static_assert(sizeof(DevEvent) == 64, "DevEvent is one 64-byte line");
 
 // Device buffers of a launch over n_sims simulations (plain runs: 1).
 // Per simulation s: histogram count[GS_MAX_BINS], digits[GS_MAX_WEIGHTS][2]

@@ -23,7 +23,7 @@ __device__ __forceinline__ void emit(const DevOut &O, const Event &ev, uint64_t 
     if (i < O.ev_cap[sim]) {
         DevEvent d;
         d.t_inj = ev.t_inj; d.t_exit = ev.t_exit; d.r = ev.r; d.theta = ev.theta;
-        d.sumac = ev.sumac; d.id = pid; d.sim = sim; d.pad = 0;
+        d.sumac = ev.sumac; d.id = pid; d.sim = sim; d.pad = 0; d.pad2 = 0;
         O.events[O.ev_off[sim] + i] = d;
     }
 }

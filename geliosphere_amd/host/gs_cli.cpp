@@ -1,0 +1,161 @@
+// gs_cli.cpp - command-line driver for the drop-in GPU models.
+//
+// Accepts the reference CLI's model and parameter flags
+// (Input/src/ParseParams.cpp:29-62) and converts them into carrier keys the
+// way the reference's InputValidation does (K0 in cm^2/s * 4.4683705e-27,
+// V in km/s * 6.68458712e-9, N in millions), then runs the model's
+// Algorithm step: simulation, Results pass, removal of log.dat.
+//
+//   gs_cli (-F|-B|-E|-T|-b FILE) [-N millions] [-K K0] [-V V] [-d dt]
+//          [-p destination] [-c] [--evaluation log.dat]
+//          [--r AU] [--theta deg] [--tilt deg] [--polarity +-1]
+//          [--seed N] [--particles N] [--keep-log] [--no-log]
+//          [--solarprop-gpu-compat] [--device N] [--print-params]
+//
+// Defaults are those of the reference's shipped Settings.toml.
+// --print-params prints the parameter block the model would run with (JSON)
+// and exits without touching a device.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "gs_dropin.hpp"
+
+namespace {
+
+// The defaults the reference application loads from its Settings.toml
+// (Input/src/TomlSettings.cpp:13-52 reading Settings.toml); the carrier keeps
+// values already set by flags.
+void put_settings_defaults(ParamsCarrier *pc) {
+    pc->putFloat("K0_default", (float)(5e22 * 4.4683705e-27));
+    pc->putFloat("V_default", (float)(400.0 * 6.68458712e-9));
+    pc->putFloat("dt_default", 50.0f);
+    pc->putFloat("r_injection", 1.0f);
+    pc->putFloat("uniform_energy_injection_maximum", 101.0f);
+    pc->putFloat("theta_injection", 90.0f);
+    pc->putInt("use_uniform_injection", 0);
+    pc->putFloat("solarPropRatio", 0.02f);
+    pc->putFloat("geliosphereRatio", 0.2f);
+    pc->putFloat("K0_ratio", 1.0f);
+    pc->putFloat("C_delta", 8.7e-5f);
+    pc->putFloat("default_tilt_angle", 0.1f);
+    pc->putInt("remove_log_files_after_simulation", 1);
+}
+
+void usage() {
+    std::fprintf(stderr,
+                 "usage: gs_cli (-F|-B|-E|-T|-b FILE) [-N millions] [-K K0_cm2s] [-V V_kms] [-d dt]\n"
+                 "              [-p destination] [-c] [--evaluation LOG] [--r AU] [--theta deg]\n"
+                 "              [--tilt deg] [--polarity 1|-1] [--seed N] [--particles N]\n"
+                 "              [--keep-log] [--no-log] [--solarprop-gpu-compat] [--device N]\n"
+                 "              [--print-params]\n");
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    ParamsCarrier *pc = ParamsCarrier::instance();
+    int model = -1;
+    std::string batch;
+    bool print_params = false;
+    for (int i = 1; i < argc; ++i) {
+        const std::string a = argv[i];
+        auto val = [&]() -> const char * {
+            if (i + 1 >= argc) {
+                usage();
+                std::exit(2);
+            }
+            return argv[++i];
+        };
+        if (a == "-F" || a == "--forward") model = GS_MODEL_1D_FP;
+        else if (a == "-B" || a == "--backward") model = GS_MODEL_1D_BP;
+        else if (a == "-E" || a == "--solarprop-like-model") model = GS_MODEL_2D_SOLARPROP;
+        else if (a == "-T" || a == "--geliosphere-2d-model") model = GS_MODEL_2D_GELIOSPHERE;
+        else if (a == "-b" || a == "--batchrun") batch = val();
+        else if (a == "-c" || a == "--csv") pc->putInt("csv", 1);
+        else if (a == "-N" || a == "--number-of-test-particles") {
+            const int n = std::atoi(val());
+            if (n <= 0) { std::fprintf(stderr, "Number of test particles must be greater than 0!\n"); return 2; }
+            pc->putInt("millions", n);
+        } else if (a == "-K" || a == "--K0") {
+            const float K = (float)std::atof(val());
+            if (K < 0.0f) { std::fprintf(stderr, "K0 is out of range!(>0)\n"); return 2; }
+            pc->putFloat("K0", (float)(K * 4.4683705e-27));
+            pc->putInt("K0_entered_by_user", 1);
+        } else if (a == "-V" || a == "--V") {
+            const float V = (float)std::atof(val());
+            if (V < 100.0f || V > 1500.0f) { std::fprintf(stderr, "V is out of range!(100-1500 km/s)\n"); return 2; }
+            pc->putFloat("V", (float)(V * 6.68458712e-9));
+        } else if (a == "-d" || a == "--dt") {
+            const float dt = (float)std::atof(val());
+            if (dt < 3.0f || dt > 5000.0f) { std::fprintf(stderr, "dt is out of range!(3-5000)\n"); return 2; }
+            pc->putFloat("dt", dt);
+        } else if (a == "-p" || a == "--path") pc->putString("destination", val());
+        else if (a == "--evaluation") {
+            pc->putInt("run_simulation", 0);
+            pc->putString("pathToLogFile", val());
+        } else if (a == "--r") pc->putFloat("r_injection", (float)std::atof(val()));
+        else if (a == "--theta") pc->putFloat("theta_injection", (float)std::atof(val()));
+        else if (a == "--tilt") pc->putFloat("tilt_angle", (float)std::atof(val()));
+        else if (a == "--polarity") pc->putInt("polarity", std::atoi(val()));
+        else if (a == "--seed") pc->putString("gs_seed", val());
+        else if (a == "--particles") pc->putInt("gs_particles", std::atoi(val()));
+        else if (a == "--keep-log") pc->putInt("remove_log_files_after_simulation", 0);
+        else if (a == "--no-log") pc->putInt("gs_write_log", 0);
+        else if (a == "--solarprop-gpu-compat") pc->putInt("gs_solarprop_gpu_compat", 1);
+        else if (a == "--device") pc->putInt("gs_device", std::atoi(val()));
+        else if (a == "--print-params") print_params = true;
+        else {
+            usage();
+            return 2;
+        }
+    }
+    put_settings_defaults(pc);
+    if (!batch.empty()) {
+        if (model >= 0) { std::fprintf(stderr, "-b excludes the model flags\n"); return 2; }
+        std::vector<gsdrop::BatchRow> rows;
+        std::string err;
+        if (!gsdrop::read_batch_csv(batch, &rows, &err)) {
+            std::fprintf(stderr, "[error] %s\n", err.c_str());
+            return 1;
+        }
+        if (print_params) {
+            for (const gsdrop::BatchRow &r : rows)
+                std::printf("{\"model\": \"%s\", \"K0\": %.17g, \"V\": %.17g, \"dt\": %.17g, \"N\": %ld, "
+                            "\"r\": %.17g, \"theta\": %.17g, \"name\": \"%s\", \"year\": %d, \"month\": %d}\n",
+                            r.model.c_str(), r.K0, r.V, r.dt, r.N, r.r, r.theta, r.name.c_str(), r.year, r.month);
+            return 0;
+        }
+        const int n = gsdrop::run_batch_rows(pc, rows);
+        std::fprintf(stderr, "[info] %d of %zu batch simulations ran.\n", n, rows.size());
+        return n == (int)rows.size() ? 0 : 1;
+    }
+    if (model < 0) {
+        std::fprintf(stderr, "At least one model must be selected!\n");
+        usage();
+        return 2;
+    }
+    pc->putString("model", gsdrop::model_name(model));
+    if (print_params) {
+        gsdrop::RunConfig cfg;
+        std::string err;
+        if (!gsdrop::config_from_carrier(pc, model, &cfg, &err)) {
+            std::fprintf(stderr, "[error] %s\n", err.c_str());
+            return 1;
+        }
+        const gs_params &p = cfg.params;
+        std::printf("{\"model\": %d, \"injection\": %d, \"flags\": %u, \"K0\": %.17g, \"V\": %.17g, "
+                    "\"dt\": %.17g, \"r_inj\": %.17g, \"theta_inj\": %.17g, \"tilt\": %.17g, "
+                    "\"polarity\": %.17g, \"ratio\": %.17g, \"delta0\": %.17g, \"T_min\": %.17g, "
+                    "\"T_max\": %.17g, \"grid_n\": %u, \"n_particles\": %llu, \"seed\": %llu, "
+                    "\"n_bins\": %u, \"n_weights\": %u, \"weights\": [%d, %d, %d, %d]}\n",
+                    p.model, p.injection, p.flags, p.K0, p.V, p.dt, p.r_inj, p.theta_inj, p.tilt, p.polarity,
+                    p.ratio, p.delta0, p.T_min, p.T_max, p.grid_n, (unsigned long long)p.n_particles,
+                    (unsigned long long)p.seed, p.n_bins, p.n_weights, p.weights[0], p.weights[1],
+                    p.weights[2], p.weights[3]);
+        return 0;
+    }
+    gsdrop::run_algorithm(model, pc);
+    return 0;
+}

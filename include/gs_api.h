@@ -203,7 +203,7 @@ gs_status gs_run_plain(gs_ctx *ctx, const gs_params *p, gs_spectrum *out,
  * run alone.  gs_last_run_stats covers the whole batch. */
 gs_status gs_run_batch(gs_ctx *ctx, const gs_params *sims, uint32_t n_sims,
                        gs_spectrum *outs);
-/* Event records (56 B each) a batch may hold on the device at once
+/* Event records (64 B each) a batch may hold on the device at once
  * (default 2^29); larger batches run as consecutive sub-batches. */
 gs_status gs_set_batch_budget(gs_ctx *ctx, uint64_t max_events);
 

@@ -1,0 +1,193 @@
+"""Host side of the drop-in GPU models, on the CPU (no device calls).
+
+* ParamsCarrier (standalone build) behaves like the reference's
+  (Input/include/ParamsCarrier.hpp:158-300);
+* gs_cli maps the reference's flags and carrier keys onto the parameter
+  block the way the reference sets its constants
+  (CUDAKernel/src/CosmicConstants.cu:43-111, Input/src/InputValidation.cpp:57-118):
+  float arithmetic, K0 used for 1D models only when entered by the user,
+  Geliosphere K0 times K0_ratio, V sign for B-p, particle counts rounded up
+  to whole 4096 x 256 launches;
+* the Results passes (--evaluation) bin and weight log.dat lines and write
+  the spectrum files in the reference's formats
+  (Algorithm/src/*Results.cpp, Utils/src/ResultsUtils.cpp).
+"""
+import json
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "geliosphere_amd", "gs_cli")
+
+M0, Q, C = 1.67261e-27, 1.60219e-19, 2.99793e8
+T0W = M0 * C * C
+T0 = M0 * C * C / (Q * 1e9)
+f32 = np.float32
+
+
+def cli(*args, cwd=None, check=True):
+    if not os.path.exists(CLI):
+        pytest.fail("gs_cli is not built; run __graft_entry__.build()")
+    r = subprocess.run([CLI, *args], cwd=cwd, capture_output=True, text=True, timeout=120)
+    if check and r.returncode != 0:
+        raise AssertionError("gs_cli %s failed (%d): %s" % (" ".join(args), r.returncode, r.stderr))
+    return r
+
+
+def params(*args):
+    return json.loads(cli(*args, "--print-params").stdout)
+
+
+def test_params_carrier_semantics(tmp_path):
+    exe = tmp_path / "pc_check"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "geliosphere_amd", "host", "compat"),
+                    os.path.join(ROOT, "tests", "cpp", "params_carrier_check.cpp"), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=30)
+    assert out.returncode == 0 and out.stdout.strip() == "ok", out.stderr
+
+
+def conv_k0(k_cm2s):
+    return float(f32(float(f32(k_cm2s)) * 4.4683705e-27))
+
+
+def conv_v(v_kms):
+    return float(f32(float(f32(v_kms)) * 6.68458712e-9))
+
+
+def deg_f32(x):
+    return float((f32(x) * f32(3.1415926535)) / f32(180.0))
+
+
+def test_cli_maps_1d_backward_like_the_reference():
+    p = params("-B", "-K", "5e22", "-V", "400", "-d", "500", "--seed", "7")
+    assert p["model"] == 1 and p["seed"] == 7
+    assert p["K0"] == conv_k0(5e22)
+    assert p["V"] == conv_v(400)           # magnitude; the engine applies -V for B-p
+    assert p["dt"] == 500.0
+    # 1D injection: the 1e-4 GeV grid up to 101 GeV (CosmicUtils.cu:17-34)
+    assert p["injection"] == 1 and p["grid_n"] == 1010000 and p["T_min"] == 1e-4
+    assert p["n_particles"] == 1 << 20     # 1 million rounded up to one launch
+    assert p["weights"][:2] == [1, 0] and p["n_weights"] == 2 and p["n_bins"] == 30
+
+
+def test_cli_1d_k0_only_when_entered_by_user():
+    # Without -K the 1D GPU models keep their built-in K0 (CosmicConstants.cu:16,69-73)
+    assert params("-B", "--seed", "1")["K0"] == float(f32(0.000222))
+    assert params("-F", "--seed", "1")["K0"] == float(f32(0.000222))
+    # defaults of Settings.toml: dt 50 s, V 400 km/s
+    p = params("-F", "--seed", "1")
+    assert p["dt"] == 50.0 and p["V"] == conv_v(400)
+
+
+def test_cli_2d_models():
+    g = params("-T", "--seed", "3", "--theta", "45", "--tilt", "10", "--polarity", "-1")
+    assert g["model"] == 3 and g["injection"] == 2          # SolarProp energies
+    assert g["K0"] == float(f32(f32(conv_k0(5e22)) * f32(1.0)))   # table K0 x K0_ratio
+    assert g["theta_inj"] == deg_f32(45) and g["tilt"] == deg_f32(10) and g["polarity"] == -1.0
+    assert g["ratio"] == float(f32(0.2)) and g["delta0"] == float(f32(8.7e-5))
+    assert g["weights"][:3] == [2, 3, 4]
+    g_user = params("-T", "--seed", "3", "-K", "1e22")
+    assert g_user["K0"] == conv_k0(1e22)                      # user K0 is not scaled
+    e = params("-E", "--seed", "3", "--tilt", "30")
+    assert e["tilt"] == deg_f32(30) and e["flags"] == 0 and e["ratio"] == float(f32(0.02))
+    eg = params("-E", "--seed", "3", "--tilt", "30", "--solarprop-gpu-compat")
+    assert eg["tilt"] == deg_f32(5.75) and eg["flags"] == 1
+
+
+def test_cli_particle_counts_and_range_checks():
+    assert params("-B", "-N", "3", "--seed", "1")["n_particles"] == 3 * (1 << 20)
+    assert params("-B", "--particles", "777", "--seed", "1")["n_particles"] == 777
+    assert cli("-B", "-V", "50", check=False).returncode != 0         # V range 100-1500 km/s
+    assert cli("-B", "-d", "1", check=False).returncode != 0          # dt range 3-5000 s
+    assert cli("-B", "-N", "0", check=False).returncode != 0
+    assert cli(check=False).returncode != 0                           # no model
+
+
+def test_batch_csv_rows(tmp_path):
+    f = tmp_path / "batch.csv"
+    f.write_text("year,month,K0,V,dt,N,r,theta,pathToCustomSettingsFile,name,model\n"
+                 "-1,-1,5e22,400,500,1,1.0,90,,runA,1D Bp\n"
+                 "2010,5,,,,2,30,45,,\"run B\",2D Geliosphere\n")
+    rows = [json.loads(l) for l in cli("-b", str(f), "--print-params").stdout.splitlines()]
+    assert rows[0]["model"] == "1D Bp" and rows[0]["K0"] == 5e22 and rows[0]["dt"] == 500 and rows[0]["name"] == "runA"
+    assert rows[1]["K0"] == -1 and rows[1]["N"] == 2 and rows[1]["name"] == "run B" and rows[1]["year"] == 2010
+
+
+# ---------------------------------------------------------------- Results passes
+
+def p_si(T):
+    tw = T * 1e9 * Q
+    return math.sqrt(tw * (tw + 2 * T0W)) / Q * Q / C
+
+
+def yamada(p):
+    return math.exp(-1.85 * math.log(M0 * M0 * C ** 4 + p * p * C * C)) / p
+
+
+def read_rows(path, skip_header=False):
+    lines = open(path).read().splitlines()
+    if skip_header:
+        lines = lines[1:]
+    return [[float(v) for v in l.replace(",", " ").split()] for l in lines]
+
+
+def test_results_1d_bp(tmp_path):
+    # boundary energy, 1 AU energy, r, (ignored) weight; the last line is not read
+    recs = [(2.0, 1.5, 100.01, 0.0), (0.7, 0.31, 100.2, 0.0), (3.0, 1.52, 100.1, 0.0), (60.0, 50.2, 100.0, 0.0)]
+    (tmp_path / "in.dat").write_text("".join("%g %g %g %g\n" % r for r in recs))
+    cli("-B", "--evaluation", "in.dat", cwd=tmp_path)
+    e2 = read_rows(tmp_path / "output_1e2bin.dat")
+    e3 = read_rows(tmp_path / "output_1e3bin.dat")
+    e4 = read_rows(tmp_path / "output_4e2bin.dat")
+    lg = read_rows(tmp_path / "output_logbin.dat")
+    assert (len(e2), len(e3), len(e4), len(lg)) == (100, 1000, 400, 24)
+    assert all(len(r) == 3 for r in e2) and all(len(r) == 2 for r in lg)
+    w = [yamada(p_si(tb)) * p_si(t1) ** 2 for tb, t1, _, _ in recs]
+    # trunc(T) bins: 1.5 and 1.52 -> bin 1 (row 0), 0.31 -> none
+    assert e2[0][0] == 1.0 and e2[0][1] == pytest.approx(w[0] + w[2], rel=1e-12)
+    assert e2[0][2] == pytest.approx((w[0] + w[2]) / 2, rel=1e-12)
+    assert e2[49][1] == 0.0                       # the dropped last line (50.2 GeV)
+    # trunc((T + 0.05) * 10): 1.5 -> 15, 1.52 -> 15, 0.31 -> 3
+    assert e3[14][1] == pytest.approx(w[0] + w[2], rel=1e-12) and e3[2][1] == pytest.approx(w[1], rel=1e-12)
+    # trunc((T + 0.125) * 4): 1.5 -> 6, 0.31 -> 1
+    assert e4[5][1] == pytest.approx(w[0] + w[2], rel=1e-12) and e4[0][1] == pytest.approx(w[1], rel=1e-12)
+    cli("-B", "--evaluation", "in.dat", "-c", cwd=tmp_path)
+    head = open(tmp_path / "output_1e2bin.csv").readline().strip()
+    assert head == "Tkin,spe,speAvg,speCount"
+    assert open(tmp_path / "output_logbin.csv").readline().strip() == "binc,spelog/binw"
+
+
+def test_results_1d_fp_zero_weight_inside_reflection_radius(tmp_path):
+    pinj = p_si(2.0)
+    recs = [(pinj, p_si(1.2), 1.0, 0.0, 0.01), (pinj, p_si(1.3), 0.2, 0.0, 0.01), (pinj, p_si(9.0), 1.0, 0.0, 0.0)]
+    (tmp_path / "in.dat").write_text("".join(" %g  %g  %g  %g %g \n" % r for r in recs))
+    cli("-F", "--evaluation", "in.dat", cwd=tmp_path)
+    e2 = read_rows(tmp_path / "output_1e2bin.dat")
+    # first two records bin at trunc(T) = 1; the one inside r < 0.3 carries 0
+    p100 = float("%g" % pinj)
+    p1 = float("%g" % p_si(1.2))
+    w = yamada(p100) / 1e45 * p1 * p1 * math.exp(0.01)
+    assert e2[0][1] == pytest.approx(w, rel=1e-9)
+    assert e2[0][2] == pytest.approx(w / 2, rel=1e-9)    # two events, one with zero weight
+
+
+def test_results_2d_files(tmp_path):
+    sp = [0.01 * 1.5 ** k for k in range(30)]
+    recs = [(sp[5], 3.0, 100.0, 0.0, 1.5708, 1.0), (sp[5], 4.0, 100.0, 0.0, 1.5708, 2.0),
+            (0.2, 1.0, 100.0, 0.0, 1.5708, 1.0), (1.0, 2.0, 100.0, 0.0, 1.5708, 1.0),
+            (sp[9], 5.0, 100.0, 0.0, 1.5708, 1.0)]
+    (tmp_path / "in.dat").write_text("".join("%g %g %g %g %g %g\n" % r for r in recs))
+    cli("-T", "--evaluation", "in.dat", cwd=tmp_path)
+    for name, n in (("Usoskin_2005", 29), ("Weber", 29), ("Burger", 29), ("Ulysses", 2), ("Ams", 70)):
+        rows = read_rows(tmp_path / (name + ".dat"))
+        assert len(rows) == n and all(len(r) == 4 for r in rows), name
+    weber = read_rows(tmp_path / "Weber.dat")
+    # SolarProp energy index 6 (0.0759375 GeV) holds the first two records
+    assert weber[5][3] == 2.0 and weber[5][0] == pytest.approx(0.0759375, abs=1e-4)
+    assert sum(r[3] for r in weber) == 2.0        # the last line (index 10) is not read
+    uly = read_rows(tmp_path / "Ulysses.dat")
+    assert uly[0][3] == 1.0 and uly[1][3] == 1.0   # 0.2 GeV in (0.125, 0.25); 1.0 in (0.25, 2)
