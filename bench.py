@@ -1,14 +1,21 @@
-"""Benchmark of the plain trajectory engine on one MI355X.
+"""Benchmark of the trajectory engine on one MI355X.
 
-Workload (one "step"): the 1D backward-in-time (-B) and forward-in-time (-F)
-models, N = 1e8 particles each, K0 = 5e22 cm^2/s, V = 400 km/s, dt = 500 s,
-injection log-uniform 0.01-100 GeV, 30 log bins, FP64.  Inputs are a
-parameter block; nothing is transferred per step except the 30-bin result.
+Headline (value): plain Monte Carlo, one "step" = the 1D backward-in-time
+(-B) and forward-in-time (-F) models, N = 1e8 particles each, K0 = 5e22
+cm^2/s, V = 400 km/s, dt = 500 s, injection log-uniform 0.01-100 GeV, 30 log
+bins, FP64.  Inputs are a parameter block; nothing is transferred per step
+except the 30-bin result.  Reported as particle-steps/s (whole job), with
+the roofline of the dominant kernel (1D B-p trajectory kernel, FP64-VALU
+bound; algorithmic FLOPs per step from the oracle's op count) and the CPU
+baseline (the oracle on the host cores, bounded sample).
 
-Prints one JSON line: particle-steps/s (whole job), the roofline of the
-dominant kernel (1D B-p trajectory kernel, FP64-VALU bound; algorithmic FLOPs
-per step from the oracle's op count) and the CPU baseline (the oracle on the
-host cores, bounded sample).
+Also measured in the same run (detail):
+  batched  - spectra/s of gs_run_batch vs sequential gs_run_plain on the
+             same heterogeneous simulations (batch sampler, particle counts
+             scaled by --batch-scale), with a bit-identity check;
+  surrogate - the trained ensemble's inference rate (65,536 queries), its
+             median per-bin error over 0.1-10 GeV on the held-out Monte Carlo
+             spectra, and the plain Monte Carlo spectra/s it replaces.
 """
 import argparse
 import json
@@ -38,6 +45,10 @@ def parse():
                     help="models run per step (the headline workload is bp+fp)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
+    ap.add_argument("--batch-sims", type=int, default=64, help="simulations in the batched leg (0: skip)")
+    ap.add_argument("--batch-scale", type=float, default=0.01,
+                    help="particle counts of the batched leg relative to the batch sampler's 1e5-1e7")
+    ap.add_argument("--no-surrogate", action="store_true")
     return ap.parse_args()
 
 
@@ -86,6 +97,67 @@ def cpu_baseline(budget_s):
     return {"value": steps / t_total, "unit": "particle-steps/s", "cores": threads, "kind": "port",
             "sample": "1D B-p, %d particles (log-uniform 0.01-100 GeV, dt=500 s), %d steps in %.1f s"
                       % (sample_n, steps, t_total)}
+
+
+def batch_leg(eng, n_sims, scale, seed=2024):
+    """gs_run_batch vs one gs_run_plain per simulation, same simulations."""
+    from geliosphere_amd import batch
+    sims = batch.sample_batch(n_sims, seed=seed, n_scale=scale)
+    eng.run_batch(sims)                       # builds the graph of this layout
+    t0 = time.perf_counter()
+    outs = eng.run_batch(sims)
+    t_batch = time.perf_counter() - t0
+    st = eng.last_stats()
+    t0 = time.perf_counter()
+    same = True
+    for p, o in zip(sims, outs):
+        s, _, _ = eng.run_plain(p)
+        same &= bytes(s) == bytes(o)
+    t_seq = time.perf_counter() - t0
+    steps = sum(o.total_steps for o in outs)
+    return {"sims": n_sims, "particles": sum(p.n_particles for p in sims),
+            "particle_count_range": [int(1e5 * scale), int(1e7 * scale)],
+            "batched_spectra_per_s": n_sims / t_batch, "sequential_plain_spectra_per_s": n_sims / t_seq,
+            "speedup": t_seq / t_batch, "batched_particle_steps_per_s": steps / t_batch,
+            "lane_utilisation": st.active_cycles / max(st.lane_cycles, 1), "bit_identical_to_plain": same}
+
+
+def surrogate_leg(eng):
+    """Inference rate and held-out accuracy of the committed ensemble, and the
+    plain Monte Carlo rate at the training particle counts."""
+    import numpy as np
+    from geliosphere_amd import surrogate, surrogate_eval, surrogate_data
+    data_dir = os.path.join(ROOT, "geliosphere_amd", "data")
+    wpath = os.path.join(data_dir, "surrogate_weights.npz")
+    dpath = os.path.join(data_dir, "surrogate_dataset.npz")
+    if not (os.path.exists(wpath) and os.path.exists(dpath)):
+        return {"status": "no trained ensemble in geliosphere_amd/data"}
+    sg = surrogate.Surrogate(0)
+    sg.load(wpath)
+    d = surrogate_eval.load([dpath])
+    acc = surrogate_eval.evaluate(sg, d)
+    rate, dt = surrogate_eval.inference_rate(sg)
+    n_train = dict(zip(surrogate_data.MODELS, np.load(dpath)["n_train"].tolist()))
+    mc = {}
+    for m in surrogate_data.MODELS:
+        sims = surrogate_data.sims_for(m, 2, int(n_train[m]), 99, 777)
+        t0 = time.perf_counter()
+        for p in sims:
+            eng.run_plain(p)
+        mc[abi_name(m)] = len(sims) / (time.perf_counter() - t0)
+    mc_rate = len(mc) / sum(1.0 / v for v in mc.values())
+    return {"inference_spectra_per_s": rate, "inference_batch": 65536, "inference_batch_ms": dt * 1e3,
+            "plain_mc_spectra_per_s_at_training_n": mc, "plain_mc_spectra_per_s_mean": mc_rate,
+            "speedup_vs_plain_mc": rate / mc_rate,
+            "median_rel_err_0p1_10GeV": {k: v["median_rel_err_surrogate"] for k, v in acc.items()},
+            "median_rel_err_nearest_neighbour": {k: v["median_rel_err_nearest_neighbour"] for k, v in acc.items()},
+            "mc_reference_median_rel_se": {k: v["median_rel_se_mc_reference"] for k, v in acc.items()},
+            "fallback_share": {k: v["fallback_share"] for k, v in acc.items()}}
+
+
+def abi_name(m):
+    from geliosphere_amd import abi
+    return abi.MODEL_NAMES[m]
 
 
 def main():
@@ -160,6 +232,10 @@ def main():
             "spectra_per_s": len(models) * a.steps / wall,
         },
     }
+    if a.batch_sims > 0:
+        out["detail"]["batched"] = batch_leg(eng, a.batch_sims, a.batch_scale)
+    if not a.no_surrogate:
+        out["detail"]["surrogate"] = surrogate_leg(eng)
     eng.close()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)

@@ -20,8 +20,10 @@ from . import abi
 MODEL_CHOICES = (abi.MODEL_1D_FP, abi.MODEL_1D_BP, abi.MODEL_2D_SOLARPROP, abi.MODEL_2D_GELIOSPHERE)
 
 
-def sample_one(rng, sim_seed, n_scale=1.0, n_range=(1e5, 1e7)):
-    model = rng.choice(MODEL_CHOICES)
+def sample_one(rng, sim_seed, n_scale=1.0, n_range=(1e5, 1e7), model=None):
+    """One simulation; `model` fixes the model instead of drawing it."""
+    drawn = rng.choice(MODEL_CHOICES)
+    model = drawn if model is None else model
     two_d = model in (abi.MODEL_2D_SOLARPROP, abi.MODEL_2D_GELIOSPHERE)
     K0 = 10.0 ** rng.uniform(22.0, 23.0)
     V = 400.0 if two_d else rng.uniform(300.0, 800.0)

@@ -252,8 +252,9 @@ __device__ void fold_one(const DevParams &P, uint64_t *lds, const DevEvent &e) {
 }
 
 // ------------------------------------------------------------------ models
-// Step outcome
-enum { STEP_CONTINUE = 0, STEP_EVENT = 1, STEP_EVENT_DONE = 2, STEP_DONE = 3 };
+// A step returns whether the particle goes on; events are handed to
+// emit(t_inj, t_exit, r, theta, sumac) where they happen, so that nothing of
+// an event stays live in registers on the common path.
 
 // 1D B-p.  State: r [AU], P = pc/q [GV], beta (of the last computed T, which
 // a reflection does not restore, as in OneDimensionBpCpuModel.cpp:108-140).
@@ -269,33 +270,33 @@ struct Bp {
         r = Pm.r_inj;
         return r < 100.0002;
     }
-    __device__ __forceinline__ int step(const StepParams &Pm, const double *z, Event &ev) {
-        const double K2dt = Pm.two_dt * (Pm.K0 * beta * P);          // 2 K dt
+    template <class Emit>
+    __device__ __forceinline__ bool step(const StepParams &Pm, const double *z, const Emit &emit) {
+        const double K2dt = Pm.k2dt * (beta * P);                     // 2 K dt
         const double inv_r = fm::rcp(r);
         const double Pn = fma(-(Pm.cp * P), inv_r, P);               // p - 2 V p dt / (3 r)
-        const double dr = fma(K2dt, inv_r, Pm.vdt) + z[0] * fm::sqrt_pos(K2dt);
+        const double dr = fma(z[0], fm::sqrt_pos(K2dt), fma(K2dt, inv_r, Pm.vdt));
         double rn = r + dr;
         const double E2 = fma(Pn, Pn, kT0 * kT0);
         const double invE = fm::rsqrt(E2);
         const double T = fma(E2, invE, -kT0);                         // sqrt(T0^2 + P^2) - T0
         const double bn = Pn * invE;
-        int out = STEP_CONTINUE;
+        bool alive = true;
         if (bn > kBetaMinBp && T < 200.0) {
             if (rn > 100.0 && (rn - dr) < 100.0) {
-                ev.t_inj = t_inj; ev.t_exit = T; ev.r = rn; ev.theta = 0.0; ev.sumac = 0.0;
+                emit(t_inj, T, rn, 0.0, 0.0);
                 r = rn;
-                return STEP_EVENT_DONE;
+                return false;
             }
         } else if (bn < kBetaMinBp) {
-            out = STEP_DONE;
+            alive = false;
         }
         beta = bn;
         const bool refl = rn < kReflectBp;
         rn = refl ? rn - dr : rn;     // a reflected step keeps P
         P = refl ? P : Pn;
         r = rn;
-        if (!(rn < 100.0002)) out = STEP_DONE;
-        return out;
+        return alive && rn < 100.0002;
     }
 };
 
@@ -314,24 +315,21 @@ struct Fp {
         r = 100.0001;
         return true;
     }
-    __device__ __forceinline__ int step(const StepParams &Pm, const double *z, Event &ev) {
-        const double K2dt = Pm.two_dt * (Pm.K0 * beta * R);
+    template <class Emit>
+    __device__ __forceinline__ bool step(const StepParams &Pm, const double *z, const Emit &emit) {
+        const double K2dt = Pm.k2dt * (beta * R);
         const double inv_r = fm::rcp(r);
-        const double dr = fma(K2dt, inv_r, Pm.vdt) + z[0] * fm::sqrt_pos(K2dt);
+        const double dr = fma(z[0], fm::sqrt_pos(K2dt), fma(K2dt, inv_r, Pm.vdt));
         const double Pn = fma(-(Pm.cp * P), inv_r, P);
         sumac = fma(Pm.csum, inv_r, sumac);
         double rn = r + dr;
         const double E2 = fma(Pn, Pn, kT0 * kT0);
         const double invE = fm::rsqrt(E2);
         const double T = fma(E2, invE, -kT0);
-        int out = STEP_CONTINUE;
         if (beta > 0.01 && T < 100.0) {
             // (r - 1) / ((r - dr) - 1) < 0, without the division
             const double a = rn - 1.0, b = (rn - dr) - 1.0;
-            if ((a < 0.0 && b >= 0.0) || (a > 0.0 && b < 0.0)) {
-                ev.t_inj = t_inj; ev.t_exit = T; ev.r = rn; ev.theta = 0.0; ev.sumac = sumac;
-                out = STEP_EVENT;
-            }
+            if ((a < 0.0 && b >= 0.0) || (a > 0.0 && b < 0.0)) emit(t_inj, T, rn, 0.0, sumac);
         }
         const bool stop = beta < 0.01;
         beta = Pn * invE;
@@ -340,8 +338,7 @@ struct Fp {
         rn = refl ? r : rn;           // a reflected step keeps r and P
         P = refl ? P : Pn;
         r = rn;
-        if (stop || !(rn < 100.0002)) out = (out == STEP_EVENT) ? STEP_EVENT_DONE : STEP_DONE;
-        return out;
+        return !stop && rn < 100.0002;
     }
 };
 
@@ -357,7 +354,8 @@ struct Sp {
         theta = Pm.theta_inj;
         return r < 100.0;
     }
-    __device__ __forceinline__ int step(const StepParams &Pm, const double *z, Event &ev) {
+    template <class Emit>
+    __device__ __forceinline__ bool step(const StepParams &Pm, const double *z, const Emit &emit) {
         double s, c;
         sincos(theta, &s, &c);
         const double Rig = fm::sqrt_pos(T * (T + 2.0 * kT0));
@@ -414,10 +412,10 @@ struct Sp {
         T = T - dTkin;
         if (r < 0.0) r = -r;
         if (r > 100.0) {
-            ev.t_inj = t_inj; ev.t_exit = T; ev.r = r; ev.theta = theta; ev.sumac = 0.0;
-            return STEP_EVENT_DONE;
+            emit(t_inj, T, r, theta, 0.0);
+            return false;
         }
-        return (r < 100.0) ? STEP_CONTINUE : STEP_DONE;
+        return r < 100.0;
     }
 };
 
@@ -434,7 +432,8 @@ struct Ge {
         theta = Pm.theta_inj;
         return r < 100.0;
     }
-    __device__ __forceinline__ int step(const StepParams &Pm, const double *z, Event &ev) {
+    template <class Emit>
+    __device__ __forceinline__ bool step(const StepParams &Pm, const double *z, const Emit &emit) {
         constexpr double kf = 1.0 / (3.0 * (5.0 / 3.4));   // as written in GeliosphereCpuModel.cpp:210-299
         double s, c;
         sincos(theta, &s, &c);
@@ -532,10 +531,10 @@ struct Ge {
             theta = (2.0 * kPi) - theta;
         if (r < 0.0) r = -r;
         if (r > 100.0) {
-            ev.t_inj = t_inj; ev.t_exit = T; ev.r = r; ev.theta = theta; ev.sumac = 0.0;
-            return STEP_EVENT_DONE;
+            emit(t_inj, T, r, theta, 0.0);
+            return false;
         }
-        return (r < 100.0) ? STEP_CONTINUE : STEP_DONE;
+        return r < 100.0;
     }
 };
 

@@ -18,12 +18,13 @@ namespace gs {
 
 // Append one event to its simulation's event region (counted even when the
 // region is full; the host then runs the simulation again with more room).
-__device__ __forceinline__ void emit(const DevOut &O, const Event &ev, uint64_t pid, uint32_t sim) {
+__device__ __forceinline__ void emit_event(const DevOut &O, uint64_t pid, uint32_t sim, double t_inj, double t_exit,
+                                        double r, double theta, double sumac) {
     const unsigned long long i = atomicAdd((unsigned long long *)&O.n_events[sim], 1ull);
     if (i < O.ev_cap[sim]) {
         DevEvent d;
-        d.t_inj = ev.t_inj; d.t_exit = ev.t_exit; d.r = ev.r; d.theta = ev.theta;
-        d.sumac = ev.sumac; d.id = pid; d.sim = sim; d.pad = 0; d.pad2 = 0;
+        d.t_inj = t_inj; d.t_exit = t_exit; d.r = r; d.theta = theta;
+        d.sumac = sumac; d.id = pid; d.sim = sim; d.pad = 0; d.pad2 = 0;
         O.events[O.ev_off[sim] + i] = d;
     }
 }
@@ -72,15 +73,18 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
     Model m;
     uint64_t pid = 0;
     uint32_t steps = 0, blk = 0;
-    bool active = false;
+    bool active = false, has_particle = false;
     uint64_t q_next = 0, q_end = 0;
     bool exhausted = false;
     uint64_t my_steps = 0;
     uint32_t my_particles = 0, my_nan = 0, my_cap = 0, cycles = 0;
 
-    auto do_step = [&](const double *z, Event &ev) -> int {
-        if constexpr (kBatch) return m.step(lp, z, ev);
-        else return m.step(P0, z, ev);
+    auto emit = [&](double t_inj, double t_exit, double r, double theta, double sumac) {
+        emit_event(O, pid, sim, t_inj, t_exit, r, theta, sumac);
+    };
+    auto do_step = [&](const double *z) -> bool {
+        if constexpr (kBatch) return m.step(lp, z, emit);
+        else return m.step(P0, z, emit);
     };
 
     for (;;) {
@@ -113,7 +117,13 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
                 q_next += (uint64_t)n_idle;
             }
             if (take != ~0ull) {
-                my_steps += steps;
+                // the lane's previous particle ended: its steps, and whether
+                // it stopped on a NaN state
+                if (has_particle) {
+                    my_steps += steps;
+                    if (isnan(m.r)) ++my_nan;
+                }
+                has_particle = true;
                 steps = 0;
                 blk = 0;
                 double T;
@@ -162,20 +172,25 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
             // normal's position is known at compile time.
             double z[4];
             normal_block(pid, blk, k0, k1, z);
+            if (__ballot(active && steps + kCycleSteps > cap) == 0) {
+                // no lane can reach its step cap in this cycle
 #pragma unroll
-            for (int s = 0; s < kCycleSteps; ++s) {
-                if (active) {
-                    if (steps >= cap) {
-                        ++my_cap;
-                        active = false;
-                    } else {
-                        Event ev;
-                        const int rc = do_step(&z[s], ev);
+                for (int s = 0; s < kCycleSteps; ++s) {
+                    if (active) {
                         ++steps;
-                        if (rc == STEP_EVENT || rc == STEP_EVENT_DONE) emit(O, ev, pid, sim);
-                        if (rc >= STEP_EVENT_DONE) {
+                        if (!do_step(&z[s])) active = false;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < kCycleSteps; ++s) {
+                    if (active) {
+                        if (steps >= cap) {
+                            ++my_cap;
                             active = false;
-                            if (rc == STEP_DONE && isnan(m.r)) ++my_nan;
+                        } else {
+                            ++steps;
+                            if (!do_step(&z[s])) active = false;
                         }
                     }
                 }
@@ -208,21 +223,18 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
                         ++my_cap;
                         active = false;
                     } else {
-                        Event ev;
-                        const int rc = do_step(z, ev);
                         ++steps;
-                        if (rc == STEP_EVENT || rc == STEP_EVENT_DONE) emit(O, ev, pid, sim);
-                        if (rc >= STEP_EVENT_DONE) {
-                            active = false;
-                            if (rc == STEP_DONE && isnan(m.r)) ++my_nan;
-                        }
+                        if (!do_step(z)) active = false;
                     }
                 }
             }
         }
         blk += (uint32_t)NPS;
     }
-    my_steps += steps;
+    if (has_particle) {
+        my_steps += steps;
+        if (isnan(m.r)) ++my_nan;
+    }
     if constexpr (kBatch) {
         if (my_particles) flush_sim_stats(O, sim, my_steps, my_particles, my_nan, my_cap);
         if (lane == 0) atomicAdd((unsigned long long *)O.cycles, (unsigned long long)cycles);

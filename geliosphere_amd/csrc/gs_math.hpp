@@ -3,7 +3,8 @@
 // The library versions of div, sqrt, log, exp, sin, cos round correctly or
 // handle every range (Payne-Hanek reduction, denormal scaling); the step
 // kernels only need ~1 ulp on known ranges.  These versions:
-//   rcp, rsqrt, sqrt  hardware seed (v_rcp_f64 / v_rsq_f64) + Newton steps;
+//   rcp, rsqrt, sqrt  hardware seed (v_rcp_f64 / v_rsq_f64) + Newton /
+//                     Goldschmidt steps;
 //   log               exponent split, s = (m-1)/(m+1), 2s + 2s w R(w);
 //   exp               k ln2 + r reduction (two-part ln2), Taylor to r^13;
 //   sincos_turn       sin/cos(2 pi x 2^-32) for a 32-bit integer x, with
@@ -56,22 +57,30 @@ __device__ __forceinline__ double rcp(double x) {
     return fma(e, y, y);
 }
 
-// 1/sqrt(x) for normal x > 0: v_rsq_f64 seed + 2 Newton steps.
+// 1/sqrt(x) for normal x > 0: v_rsq_f64 seed + 2 Newton steps in the form
+// y += y (1/2 - (x/2) y^2).
 __device__ __forceinline__ double rsqrt(double x) {
+    const double hx = 0.5 * x;
     double y = __builtin_amdgcn_rsq(x);
-    double r = fma(-(x * y), y, 1.0);
-    y = fma(0.5 * y, r, y);
-    r = fma(-(x * y), y, 1.0);
-    return fma(0.5 * y, r, y);
+    double e = fma(-hx, y * y, 0.5);
+    y = fma(y, e, y);
+    e = fma(-hx, y * y, 0.5);
+    return fma(y, e, y);
 }
 
-// sqrt(x) for x >= 0 (0 -> 0): s = x rsqrt(x) with one Heron correction.
+// sqrt(x) for x >= 0 (0 -> 0).  Goldschmidt: from the seed y ~ 1/sqrt(x),
+// g = x y -> sqrt(x) and h = y/2 -> 1/(2 sqrt(x)) converge together (one
+// step squares the seed's ~2^-24 error); a final residual step
+// g += (x - g^2) h rounds g to within an ulp.
 __device__ __forceinline__ double sqrt_pos(double x) {
-    const double y = rsqrt(x);
-    double s = x * y;
-    const double e = fma(-s, s, x);
-    s = fma(e, 0.5 * y, s);
-    return x > 0.0 ? s : 0.0;
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    const double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    return x > 0.0 ? g : 0.0;
 }
 
 // Ops policy: fma3 (general), fmac (addend a constant: Horner steps),

@@ -255,6 +255,7 @@ gs_status gs_surrogate_default_config(gs_surrogate_config *c) {
     c->eps = 1e-8f;
     c->weight_decay = 0.0f;
     c->std_limit = 0.01f;
+    c->std_outputs = 0xffffffffu;
     return GS_OK;
 }
 
@@ -584,7 +585,8 @@ gs_status gs_surrogate_predict(gs_surrogate *s, const float *x, uint32_t n, floa
             for (uint32_t r = 0; r < rows; ++r) {
                 uint8_t f = 0;
                 for (uint32_t c = 0; c < no; ++c)
-                    if (!(hs[(size_t)r * no + c] <= s->cfg.std_limit)) f |= GS_SG_FALLBACK_STD;
+                    if (((s->cfg.std_outputs >> c) & 1u) && !(hs[(size_t)r * no + c] <= s->cfg.std_limit))
+                        f |= GS_SG_FALLBACK_STD;
                 for (uint32_t c = 0; c < ni; ++c) {
                     const float v = x[(size_t)(r0 + r) * ni + c];
                     if (!(v >= s->lo[c] && v <= s->hi[c])) f |= GS_SG_FALLBACK_DOMAIN;

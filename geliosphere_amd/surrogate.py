@@ -31,7 +31,7 @@ class SurrogateConfig(ctypes.Structure):
                 ("hidden", ctypes.c_uint32), ("n_hidden_layers", ctypes.c_uint32), ("batch", ctypes.c_uint32),
                 ("seed", ctypes.c_uint64), ("lr", ctypes.c_float), ("beta1", ctypes.c_float),
                 ("beta2", ctypes.c_float), ("eps", ctypes.c_float), ("weight_decay", ctypes.c_float),
-                ("std_limit", ctypes.c_float)]
+                ("std_limit", ctypes.c_float), ("std_outputs", ctypes.c_uint32)]
 
 
 _bound = False
@@ -70,9 +70,22 @@ def _f32(a):
     return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
+def window_outputs(lo=0.1, hi=10.0):
+    """Bit mask of the energy bins (30 log bins, 0.01-100 GeV) inside [lo, hi] GeV:
+    the outputs whose ensemble spread decides the Monte Carlo fallback."""
+    mask = 0
+    for b in range(N_OUT):
+        e0, e1 = 0.01 * 10 ** (4 * b / 30), 0.01 * 10 ** (4 * (b + 1) / 30)
+        if e0 >= lo * (1 - 1e-12) and e1 <= hi * (1 + 1e-12):
+            mask |= 1 << b
+    return mask
+
+
 def default_config(**over):
+    """The C defaults, with the fallback std test limited to 0.1-10 GeV."""
     c = SurrogateConfig()
     lib().gs_surrogate_default_config(ctypes.byref(c))
+    c.std_outputs = window_outputs()
     for k, v in over.items():
         setattr(c, k, v)
     return c

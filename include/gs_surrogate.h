@@ -43,6 +43,8 @@ typedef struct gs_surrogate_config {
     float weight_decay;        /* decoupled (default 0)                      */
     float std_limit;           /* fallback when an output's ensemble std
                                   exceeds this (default 0.01)                */
+    uint32_t std_outputs;      /* bit i set: output i takes part in the std
+                                  test (default: all outputs)                */
 } gs_surrogate_config;
 
 /* Fallback flags of gs_surrogate_predict */

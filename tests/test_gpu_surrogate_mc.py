@@ -1,0 +1,71 @@
+"""The trained surrogate against Monte Carlo.
+
+geliosphere_amd/data holds spectra made by the batched engine (training:
+1000 per model; evaluation: 40 held-out per model with 6-10x more particles)
+and the ensemble trained on them.  The ensemble must reproduce the held-out
+Monte Carlo spectra to 2 % (median over bins in 0.1-10 GeV) for the models
+whose training spectra resolve every bin, and beat the method without a
+network: the nearest Monte Carlo spectrum of the training set.  Queries it
+does not trust must come back from Monte Carlo.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from geliosphere_amd import abi
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DATA = os.path.join(ROOT, "geliosphere_amd", "data")
+
+
+@pytest.fixture(scope="module")
+def trained():
+    from geliosphere_amd import surrogate, surrogate_eval
+    sg = surrogate.Surrogate(0)
+    sg.load(os.path.join(DATA, "surrogate_weights.npz"))
+    d = surrogate_eval.load([os.path.join(DATA, "surrogate_dataset.npz")])
+    return sg, d
+
+
+def test_accuracy_against_held_out_monte_carlo(trained):
+    from geliosphere_amd import surrogate_eval
+    sg, d = trained
+    res = surrogate_eval.evaluate(sg, d)
+    for name, r in res.items():
+        print("%-18s surrogate %.4f  nearest MC spectrum %.4f  MC reference SE %.4f  fallback %.2f"
+              % (name, r["median_rel_err_surrogate"], r["median_rel_err_nearest_neighbour"],
+                 r["median_rel_se_mc_reference"], r["fallback_share"]))
+    assert res["all"]["median_rel_err_surrogate"] < 0.02
+    for name in ("1D Bp", "2D SolarProp-like", "2D Geliosphere"):
+        assert res[name]["median_rel_err_surrogate"] < 0.02
+    for name, r in res.items():
+        assert r["median_rel_err_surrogate"] < r["median_rel_err_nearest_neighbour"], name
+
+
+def test_inference_rate(trained):
+    from geliosphere_amd import surrogate_eval
+    sg, _ = trained
+    rate, dt = surrogate_eval.inference_rate(sg, n=65536, repeats=3)
+    print("65,536 queries in %.2f ms: %.3e spectra/s" % (dt * 1e3, rate))
+    assert rate > 1e6
+
+
+def test_untrusted_queries_come_from_monte_carlo(trained, engine):
+    from geliosphere_amd import surrogate
+    sg, _ = trained
+    inside = abi.make_params(abi.MODEL_1D_BP, K0_cm2s=3e22, V_kms=500.0, r_inj=20.0, theta_deg=90.0,
+                             N=20_000, seed=5, dt=500.0)
+    outside = abi.make_params(abi.MODEL_1D_BP, K0_cm2s=3e22, V_kms=500.0, r_inj=95.0, theta_deg=90.0,
+                              N=20_000, seed=6, dt=500.0)      # r outside the 1-80 AU training box
+    y, from_mc = surrogate.query(sg, engine, [inside, outside])
+    assert from_mc[1]
+    s = engine.run_batch([outside])[0]
+    y_mc, _ = surrogate.targets(outside, s)
+    assert np.array_equal(y[1], y_mc)
+    mean, sd, fb = sg.predict(np.array([surrogate.features(inside)], dtype=np.float32))
+    assert fb[0] & surrogate.FALLBACK_DOMAIN == 0
+    if not from_mc[0]:
+        assert np.allclose(y[0], mean[0])
