@@ -5,6 +5,7 @@ relative error of each fit on its interval.
   log:  ln((1+s)/(1-s)) = 2s + 2s w R(w),  w = s^2 in [0, (3-2sqrt2)^2]
   sin:  sin a = a - a w S(w),              w = a^2 in [0, (pi/4)^2]
   cos:  cos a = 1 - w/2 + w^2 C(w)
+  atan: atan t = t - t w A(w),            w = t^2 in [0, tan(pi/16)^2]
 """
 import mpmath as mp
 
@@ -39,6 +40,16 @@ def C(w):
     return (mp.cos(a) - 1 + w / 2) / (w * w)
 
 
+T_MAX = mp.tan(mp.pi / 16)
+
+
+def A(w):
+    if w == 0:
+        return mp.mpf(1) / 3
+    t = mp.sqrt(w)
+    return (t - mp.atan(t)) / (t * w)
+
+
 def worst(kind, coeffs, n=4000):
     """Worst relative error of the final function value over the interval."""
     e = mp.mpf(0)
@@ -53,6 +64,11 @@ def worst(kind, coeffs, n=4000):
             w = a * a
             p = sum(c * w ** k for k, c in enumerate(coeffs))
             v, ref = a - a * w * p, mp.sin(a)
+        elif kind == "atan":
+            t = T_MAX * i / n
+            w = t * t
+            p = sum(c * w ** k for k, c in enumerate(coeffs))
+            v, ref = t - t * w * p, mp.atan(t)
         else:
             a = A_MAX * i / n
             w = a * a
@@ -63,7 +79,8 @@ def worst(kind, coeffs, n=4000):
 
 
 if __name__ == "__main__":
-    for kind, f, hi, deg in (("log", R, S_MAX ** 2, 6), ("sin", S, A_MAX ** 2, 5), ("cos", C, A_MAX ** 2, 5)):
+    for kind, f, hi, deg in (("log", R, S_MAX ** 2, 6), ("sin", S, A_MAX ** 2, 5), ("cos", C, A_MAX ** 2, 5),
+                             ("atan", A, T_MAX ** 2, 8)):
         c, _ = fit(f, 0, hi, deg)
         print(kind, "degree", deg, "worst rel err %.3e" % float(worst(kind, c)))
         for k, v in enumerate(c):

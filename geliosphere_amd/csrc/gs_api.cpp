@@ -724,7 +724,7 @@ gs_status gs_run_plain(gs_ctx *ctx, const gs_params *p, gs_spectrum *out, gs_exi
 
 gs_status gs_debug_math(gs_ctx *ctx, int32_t fn, const double *x, double *y, uint64_t n) {
     if (!ctx || (!x && fn != GS_DBG_NORMALS) || !y) return GS_ERR_INVALID_ARG;
-    if (fn < GS_DBG_RCP || fn > GS_DBG_RAW_SQRT) return fail(ctx, GS_ERR_INVALID_ARG, "unknown fn");
+    if (fn < GS_DBG_RCP || fn > GS_DBG_ATAN) return fail(ctx, GS_ERR_INVALID_ARG, "unknown fn");
     if (!x) return GS_ERR_INVALID_ARG;
     if (n == 0) return GS_OK;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -766,6 +766,9 @@ gs_status gs_host_math(int32_t fn, const double *x, double *y, uint64_t n) {
         case GS_DBG_SIN_TURN: fm::sincos_turn<H>((uint32_t)x[i], s, c); y[i] = s; break;
         case GS_DBG_COS_TURN: fm::sincos_turn<H>((uint32_t)x[i], s, c); y[i] = c; break;
         case GS_DBG_LOG_U32: y[i] = fm::log_u32<H>((uint32_t)x[i]); break;
+        case GS_DBG_SIN: fm::sincos_small<H>(x[i], s, c); y[i] = s; break;
+        case GS_DBG_COS: fm::sincos_small<H>(x[i], s, c); y[i] = c; break;
+        case GS_DBG_ATAN: y[i] = fm::atan_e<H>(x[i]); break;
         default: return GS_ERR_UNSUPPORTED;   // rcp / rsqrt / sqrt seeds exist on the device only
         }
     }

@@ -357,7 +357,7 @@ struct Sp {
     template <class Emit>
     __device__ __forceinline__ bool step(const StepParams &Pm, const double *z, const Emit &emit) {
         double s, c;
-        sincos(theta, &s, &c);
+        fm::sincosd(theta, s, c);
         const double Rig = fm::sqrt_pos(T * (T + 2.0 * kT0));
         const double inv_E = fm::rcp(T + kT0);
         const double beta = Rig * inv_E;
@@ -388,13 +388,16 @@ struct Sp {
         const double dTkin = -Pm.cp * ((T + 2.0 * kT0) * inv_E) * T * inv_r;
         // drifts
         const double Larmor = (0.0225 / kA) * Rig * r2 * inv_sq;
-        const double sy = sin(Pm.tilt + Larmor * (kPi / 90.0) * inv_r);
+        const double sy = fm::sind(Pm.tilt + Larmor * (kPi / 90.0) * inv_r);
         const double x = sy * fm::rcp(kPi - sy);
-        const double alphaH = acos(x);
-        const double tanH = fm::sqrt_pos(1.0 - x * x) * fm::rcp(x);
+        // the device rcp has no infinities: sqrt(1 - x^2) / 0 is set to +inf
+        const double tanH = x != 0.0 ? fm::sqrt_pos(1.0 - x * x) * fm::rcp(x) : INFINITY;
+        // acos x from tan(acos x) = sqrt(1 - x^2) / x
+        const double aH = fm::atand(tanH);
+        const double alphaH = x > 0.0 ? aH : (x < 0.0 ? kPi + aH : 0.5 * kPi);
         const double arg = (1.0 - theta * (2.0 / kPi)) * tanH;
         const double inv_aH = fm::rcp(alphaH);
-        const double f = atan(arg) * inv_aH;
+        const double f = fm::atand(arg) * inv_aH;
         const double dc = kKonvF * (2.0 / (3.0 * kA)) * Rig * beta * r * gamma * f * inv_opg * inv_opg;
         const double DriftR = Pm.polarity * dc * c * inv_s;
         const double DriftTheta = -Pm.pol_theta * dc * (2.0 + g2);
@@ -436,7 +439,7 @@ struct Ge {
     __device__ __forceinline__ bool step(const StepParams &Pm, const double *z, const Emit &emit) {
         constexpr double kf = 1.0 / (3.0 * (5.0 / 3.4));   // as written in GeliosphereCpuModel.cpp:210-299
         double s, c;
-        sincos(theta, &s, &c);
+        fm::sincosd(theta, s, c);
         const double Rig = fm::sqrt_pos(T * (T + 2.0 * kT0));
         const double inv_E = fm::rcp(T + kT0);
         const double beta = Rig * inv_E;
@@ -507,13 +510,16 @@ struct Ge {
         const double dTkin = -Pm.cp * ((T + 2.0 * kT0) * inv_E) * T * inv_r;
         // drifts (Kappl 2016 Eq. 26, 32-37 with Cb in place of 1 + gamma^2)
         const double Larmor = (0.0225 / kA) * Rig * r2 * inv_sqCb;
-        const double sy = sin(Pm.tilt + Larmor * (kPi / 90.0) * inv_r);
+        const double sy = fm::sind(Pm.tilt + Larmor * (kPi / 90.0) * inv_r);
         const double x = sy * fm::rcp(kPi - sy);
-        const double alphaH = acos(x);
-        const double tanH = fm::sqrt_pos(1.0 - x * x) * fm::rcp(x);
+        // the device rcp has no infinities: sqrt(1 - x^2) / 0 is set to +inf
+        const double tanH = x != 0.0 ? fm::sqrt_pos(1.0 - x * x) * fm::rcp(x) : INFINITY;
+        // acos x from tan(acos x) = sqrt(1 - x^2) / x
+        const double aH = fm::atand(tanH);
+        const double alphaH = x > 0.0 ? aH : (x < 0.0 ? kPi + aH : 0.5 * kPi);
         const double arg = (1.0 - theta * (2.0 / kPi)) * tanH;
         const double inv_aH = fm::rcp(alphaH);
-        const double f = atan(arg) * inv_aH;
+        const double f = fm::atand(arg) * inv_aH;
         const double dc = kKonvF * (2.0 / (3.0 * kA)) * Rig * beta * r * gamma * f * inv_Cb * inv_Cb;
         const double DriftR = Pm.polarity * dc * c * inv_s;
         const double DriftTheta = -Pm.polarity * dc * (2.0 + g2);

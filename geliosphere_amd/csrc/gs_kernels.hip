@@ -22,10 +22,18 @@ __device__ __forceinline__ void emit_event(const DevOut &O, uint64_t pid, uint32
                                         double r, double theta, double sumac) {
     const unsigned long long i = atomicAdd((unsigned long long *)&O.n_events[sim], 1ull);
     if (i < O.ev_cap[sim]) {
-        DevEvent d;
-        d.t_inj = t_inj; d.t_exit = t_exit; d.r = r; d.theta = theta;
-        d.sumac = sumac; d.id = pid; d.sim = sim; d.pad = 0; d.pad2 = 0;
-        O.events[O.ev_off[sim] + i] = d;
+        // Streaming stores: a record is written once and read by the fold
+        // pass, so it need not occupy L2 (and a lone record does not make
+        // the L2 fetch the rest of its line).
+        DevEvent *d = &O.events[O.ev_off[sim] + i];
+        __builtin_nontemporal_store(t_inj, &d->t_inj);
+        __builtin_nontemporal_store(t_exit, &d->t_exit);
+        __builtin_nontemporal_store(r, &d->r);
+        __builtin_nontemporal_store(theta, &d->theta);
+        __builtin_nontemporal_store(sumac, &d->sumac);
+        __builtin_nontemporal_store(pid, &d->id);
+        __builtin_nontemporal_store((uint64_t)sim, (uint64_t *)&d->sim);
+        __builtin_nontemporal_store((uint64_t)0, &d->pad2);
     }
 }
 
@@ -281,9 +289,10 @@ __device__ __forceinline__ void zero_lds(uint64_t *lds, int words) {
     for (int i = threadIdx.x; i < words; i += blockDim.x) lds[i] = 0;
 }
 
-template <class Model>
-__global__ void __launch_bounds__(kBlockThreads) plain_kernel(const DevParams *__restrict__ Pp,
-                                                              const DevOut O) {
+// kMinWaves: waves per SIMD asked of the register allocator (1: no request).
+template <class Model, int kMinWaves>
+__global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(kMinWaves)))
+plain_kernel(const DevParams *__restrict__ Pp, const DevOut O) {
     __shared__ uint64_t stat[ST_COUNT_];
     if (threadIdx.x < ST_COUNT_) stat[threadIdx.x] = 0;
     __syncthreads();
@@ -348,6 +357,9 @@ __global__ void debug_kernel(int fn, const double *x, double *y, uint64_t n, uin
     case GS_DBG_RAW_RCP: y[i] = __builtin_amdgcn_rcp(x[i]); break;
     case GS_DBG_RAW_RSQ: y[i] = __builtin_amdgcn_rsq(x[i]); break;
     case GS_DBG_RAW_SQRT: y[i] = __builtin_amdgcn_sqrt(x[i]); break;
+    case GS_DBG_SIN: fm::sincosd(x[i], s, c); y[i] = s; break;
+    case GS_DBG_COS: fm::sincosd(x[i], s, c); y[i] = c; break;
+    case GS_DBG_ATAN: y[i] = fm::atand(x[i]); break;
     case GS_DBG_NORMALS:
         normal_block(pid, (uint32_t)(i >> 2), (uint32_t)seed, (uint32_t)(seed >> 32), z);
         y[i] = z[i & 3];
@@ -370,15 +382,25 @@ using namespace gs;
 #ifndef GS_BATCH_WAVES_GE
 #define GS_BATCH_WAVES_GE 2
 #endif
+#ifndef GS_PLAIN_WAVES_1D
+#define GS_PLAIN_WAVES_1D 1
+#endif
+#ifndef GS_PLAIN_WAVES_GE
+#define GS_PLAIN_WAVES_GE 1
+#endif
 constexpr int kBatchWaves1D = 1, kBatchWavesSp = 2, kBatchWavesGe = GS_BATCH_WAVES_GE;
+constexpr int kPlainWaves1D = GS_PLAIN_WAVES_1D, kPlainWavesSp = 1, kPlainWavesGe = GS_PLAIN_WAVES_GE;
 
 static const void *traj_kernel_ptr(int model, bool batch) {
     switch (model) {
-    case GS_MODEL_1D_FP: return batch ? (const void *)&batch_kernel<Fp, kBatchWaves1D> : (const void *)&plain_kernel<Fp>;
-    case GS_MODEL_1D_BP: return batch ? (const void *)&batch_kernel<Bp, kBatchWaves1D> : (const void *)&plain_kernel<Bp>;
+    case GS_MODEL_1D_FP:
+        return batch ? (const void *)&batch_kernel<Fp, kBatchWaves1D> : (const void *)&plain_kernel<Fp, kPlainWaves1D>;
+    case GS_MODEL_1D_BP:
+        return batch ? (const void *)&batch_kernel<Bp, kBatchWaves1D> : (const void *)&plain_kernel<Bp, kPlainWaves1D>;
     case GS_MODEL_2D_SOLARPROP:
-        return batch ? (const void *)&batch_kernel<Sp, kBatchWavesSp> : (const void *)&plain_kernel<Sp>;
-    default: return batch ? (const void *)&batch_kernel<Ge, kBatchWavesGe> : (const void *)&plain_kernel<Ge>;
+        return batch ? (const void *)&batch_kernel<Sp, kBatchWavesSp> : (const void *)&plain_kernel<Sp, kPlainWavesSp>;
+    default:
+        return batch ? (const void *)&batch_kernel<Ge, kBatchWavesGe> : (const void *)&plain_kernel<Ge, kPlainWavesGe>;
     }
 }
 
@@ -402,14 +424,14 @@ hipError_t gs_traj_occupancy(int model, bool batch, int *blocks_per_cu) {
                                                         kBlockThreads, 0);
 }
 
-template <class Model, int kMinWaves>
+template <class Model, int kBatchWaves, int kPlainWaves>
 static void launch_traj_t(bool batch, const DevParams *sims, uint32_t n_sims, const uint64_t *offsets,
                           const DevOut &O, unsigned grid, hipStream_t s) {
     if (batch)
-        hipLaunchKernelGGL((batch_kernel<Model, kMinWaves>), dim3(grid), dim3(kBlockThreads), 0, s, sims, n_sims,
+        hipLaunchKernelGGL((batch_kernel<Model, kBatchWaves>), dim3(grid), dim3(kBlockThreads), 0, s, sims, n_sims,
                            offsets, O);
     else
-        hipLaunchKernelGGL(plain_kernel<Model>, dim3(grid), dim3(kBlockThreads), 0, s, sims, O);
+        hipLaunchKernelGGL((plain_kernel<Model, kPlainWaves>), dim3(grid), dim3(kBlockThreads), 0, s, sims, O);
 }
 
 hipError_t gs_launch_traj(int model, bool batch, const DevParams *sims, uint32_t n_sims,
@@ -418,10 +440,10 @@ hipError_t gs_launch_traj(int model, bool batch, const DevParams *sims, uint32_t
     if ((!batch && n_sims != 1) || (batch && (n_sims > (uint32_t)kBatchMaxSims || !offsets)))
         return hipErrorInvalidValue;
     switch (model) {
-    case GS_MODEL_1D_FP: launch_traj_t<Fp, kBatchWaves1D>(batch, sims, n_sims, offsets, O, grid, s); break;
-    case GS_MODEL_1D_BP: launch_traj_t<Bp, kBatchWaves1D>(batch, sims, n_sims, offsets, O, grid, s); break;
-    case GS_MODEL_2D_SOLARPROP: launch_traj_t<Sp, kBatchWavesSp>(batch, sims, n_sims, offsets, O, grid, s); break;
-    case GS_MODEL_2D_GELIOSPHERE: launch_traj_t<Ge, kBatchWavesGe>(batch, sims, n_sims, offsets, O, grid, s); break;
+    case GS_MODEL_1D_FP: launch_traj_t<Fp, kBatchWaves1D, kPlainWaves1D>(batch, sims, n_sims, offsets, O, grid, s); break;
+    case GS_MODEL_1D_BP: launch_traj_t<Bp, kBatchWaves1D, kPlainWaves1D>(batch, sims, n_sims, offsets, O, grid, s); break;
+    case GS_MODEL_2D_SOLARPROP: launch_traj_t<Sp, kBatchWavesSp, kPlainWavesSp>(batch, sims, n_sims, offsets, O, grid, s); break;
+    case GS_MODEL_2D_GELIOSPHERE: launch_traj_t<Ge, kBatchWavesGe, kPlainWavesGe>(batch, sims, n_sims, offsets, O, grid, s); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

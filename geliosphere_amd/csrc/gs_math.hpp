@@ -9,9 +9,11 @@
 //   exp               k ln2 + r reduction (two-part ln2), Taylor to r^13;
 //   sincos_turn       sin/cos(2 pi x 2^-32) for a 32-bit integer x, with
 //                     exact integer quadrant reduction (Box-Muller angle);
+//   sincos_small      sin/cos of |x| < 2^20 (Cody-Waite pi/2 reduction);
+//   atan              three-point argument reduction + odd polynomial;
 //   log_u32           ln((x + 1) 2^-32), the Box-Muller radius argument.
-// R, S, C are Chebyshev fits made by gen_poly.py (worst relative error of the
-// function value 4.5e-18, 1.4e-17, 6.9e-19 before rounding).
+// R, S, C, A are Chebyshev fits made by gen_poly.py (worst relative error of
+// the function value 4.5e-18, 1.4e-17, 6.9e-19, 3.2e-19 before rounding).
 //
 // Every multiply-add is an explicit fused multiply-add (Ops::fma3 / fmac: on
 // the device v_fma_f64, on the host std::fma).  Both are correctly rounded,
@@ -188,6 +190,55 @@ __host__ __device__ inline void sincos_turn(uint32_t x, double &s, double &c) {
     c = ((k + 1u) & 2u) ? -cc : cc;
 }
 
+// sin and cos of x for |x| < 2^20: k = rint(x 2/pi), a = x - k pi/2 with
+// pi/2 in two parts (the first has 31 fraction bits, so k * kPio2Hi is
+// exact), |a| <= pi/4, then the quadrant k mod 4.
+constexpr double kTwoOverPi = 0.6366197723675814;
+constexpr double kPio2Hi = 1.5707963267341256;
+constexpr double kPio2Lo = 6.077100506506192e-11;
+template <class Ops>
+__host__ __device__ inline void sincos_small(double x, double &s, double &c) {
+    const double k = rint(x * kTwoOverPi);
+    const double a = Ops::fma3(-k, kPio2Lo, Ops::fma3(-k, kPio2Hi, x));
+    double sa, ca;
+    sincos_poly<Ops>(a, sa, ca);
+    const uint32_t q = (uint32_t)(int32_t)k & 3u;
+    const double ss = (q & 1u) ? ca : sa;
+    const double cc = (q & 1u) ? sa : ca;
+    s = (q & 2u) ? -ss : ss;
+    c = ((q + 1u) & 2u) ? -cc : cc;
+}
+
+// atan x.  |x| > 1 uses pi/2 - atan(1/|x|); then a in [0, 1] is moved to
+// t = (a - c) / (1 + c a) with c in {0, tan(pi/8), 1}, so that
+// |t| <= tan(pi/16), and atan a = atan c + atan t with
+// atan t = t - t w A(w), w = t^2 (A: gen_poly.py, 3.2e-19).
+template <class Ops>
+__host__ __device__ inline double atan_e(double x) {
+    const double ax = fabs(x);
+    const bool big = ax > 1.0;
+    const double a = big ? (ax < INFINITY ? Ops::inv(ax) : 0.0) : ax;   // device rcp(inf) is NaN
+    const bool low = a < 0.19891236737965801;       // tan(pi/16)
+    const bool high = a >= 0.66817863791929892;     // tan(3 pi/16)
+    const double cp = high ? 1.0 : (low ? 0.0 : 0.41421356237309503);          // tan(pi/8)
+    const double base_hi = high ? 0.78539816339744828 : (low ? 0.0 : 0.39269908169872414);
+    const double base_lo = high ? 3.061616997868383e-17 : (low ? 0.0 : 1.5308084989341915e-17);
+    const double t = low ? a : (a - cp) * Ops::inv(Ops::fma3(cp, a, 1.0));
+    const double w = t * t;
+    double p = -0.05115745967981170892;
+    p = Ops::fmac(p, w, 0.06618732486766733679356);
+    p = Ops::fmac(p, w, -0.07690724472468718292484);
+    p = Ops::fmac(p, w, 0.09090879951934661083198);
+    p = Ops::fmac(p, w, -0.1111111081899807722419);
+    p = Ops::fmac(p, w, 0.1428571428427355230372);
+    p = Ops::fmac(p, w, -0.1999999999999728936999);
+    p = Ops::fmac(p, w, 0.3333333333333333249568);
+    double r = Ops::fma3(-(t * w), p, t);
+    r = base_hi + (r + base_lo);
+    if (big) r = 1.5707963267948966 - (r - 6.123233995736766e-17);
+    return x < 0.0 ? -r : r;
+}
+
 // ln((x + 1) 2^-32) for a 32-bit integer x, in [-22.18, 0].
 template <class Ops>
 __host__ __device__ inline double log_u32(uint32_t x) {
@@ -201,6 +252,13 @@ __device__ __forceinline__ double log_u32d(uint32_t x) { return log_u32<DevOps>(
 __device__ __forceinline__ void sincos_turnd(uint32_t x, double &s, double &c) {
     sincos_turn<DevOps>(x, s, c);
 }
+__device__ __forceinline__ void sincosd(double x, double &s, double &c) { sincos_small<DevOps>(x, s, c); }
+__device__ __forceinline__ double sind(double x) {
+    double s, c;
+    sincos_small<DevOps>(x, s, c);
+    return s;
+}
+__device__ __forceinline__ double atand(double x) { return atan_e<DevOps>(x); }
 
 }  // namespace fm
 }  // namespace gs

@@ -19,7 +19,7 @@ EXPORTS = ("gs_abi_version", "gs_create", "gs_destroy", "gs_last_error", "gs_def
            "gs_host_math")
 
 DBG_RCP, DBG_RSQRT, DBG_SQRT, DBG_LOG, DBG_EXP, DBG_SIN_TURN, DBG_COS_TURN, DBG_LOG_U32, \
-    DBG_RAW_RCP, DBG_RAW_RSQ, DBG_RAW_SQRT = range(11)
+    DBG_RAW_RCP, DBG_RAW_RSQ, DBG_RAW_SQRT, DBG_SIN, DBG_COS, DBG_ATAN = range(14)
 
 
 class GsError(RuntimeError):

@@ -237,6 +237,9 @@ typedef enum gs_debug_fn {
     GS_DBG_RAW_RCP = 8,   /* hardware v_rcp_f64 seed alone             */
     GS_DBG_RAW_RSQ = 9,   /* hardware v_rsq_f64 seed alone             */
     GS_DBG_RAW_SQRT = 10, /* hardware v_sqrt_f64 alone                 */
+    GS_DBG_SIN = 11,      /* sin x, |x| < 2^20                         */
+    GS_DBG_COS = 12,      /* cos x, |x| < 2^20                         */
+    GS_DBG_ATAN = 13,     /* atan x                                    */
     GS_DBG_NORMALS = 100  /* internal: normal stream (gs_debug_normals) */
 } gs_debug_fn;
 

@@ -27,10 +27,24 @@ def samples(seed=1):
     exp_x = [rng.uniform(-700, 700) for _ in range(600)] + [rng.uniform(-1, 1) for _ in range(400)] + [0.0]
     u32 = [float(rng.getrandbits(32)) for _ in range(1000)] + \
           [0.0, 1.0, 2.0 ** 29, 2.0 ** 29 - 1, 2.0 ** 30, 2.0 ** 31, 3 * 2.0 ** 30, 2.0 ** 32 - 1]
-    return log_x, exp_x, u32
+    trig = [rng.uniform(-10, 10) for _ in range(400)] + [rng.uniform(0, math.pi) for _ in range(400)] + \
+           [0.0, math.pi / 4, math.pi / 2, math.pi, 2 * math.pi, 1e-10, -3.0, 1000.5]
+    at = [s * 10.0 ** rng.uniform(-8, 8) for s in (1, -1) for _ in range(400)] + \
+         [0.0, 1.0, -1.0, 0.19891236737965801, 0.66817863791929892, 0.4142135623730950, 1e300, -1e300]
+    return log_x, exp_x, u32, trig, at
 
 
-def check(fn_eval, log_x, exp_x, u32):
+def check_trig(fn_eval, trig, at):
+    for fn, f in ((E.DBG_SIN, mpmath.sin), (E.DBG_COS, mpmath.cos)):
+        y = fn_eval(fn, trig)
+        assert max(abs(a - float(f(x))) for a, x in zip(y, trig)) <= 2.5e-16
+    y = fn_eval(E.DBG_ATAN, at)
+    assert max(ulps(a, mpmath.atan(x)) for a, x in zip(y, at)) <= 2.0
+    assert fn_eval(E.DBG_ATAN, [float("inf")])[0] == math.pi / 2
+
+
+def check(fn_eval, log_x, exp_x, u32, trig, at):
+    check_trig(fn_eval, trig, at)
     y = fn_eval(E.DBG_LOG, log_x)
     assert max(ulps(a, mpmath.log(x)) for a, x in zip(y, log_x)) <= 2.0
     y = fn_eval(E.DBG_EXP, exp_x)
@@ -52,8 +66,7 @@ def test_host_math_accuracy():
 
 @pytest.mark.gpu
 def test_device_math_accuracy(engine):
-    log_x, exp_x, u32 = samples(2)
-    check(engine.debug_math, log_x, exp_x, u32)
+    check(engine.debug_math, *samples(2))
     rng = random.Random(3)
     xs = [10.0 ** rng.uniform(-200, 200) for _ in range(1000)] + [0.3, 1.0, 99.9999]
     for fn, f in ((E.DBG_RCP, lambda v: 1 / mpmath.mpf(v)), (E.DBG_RSQRT, lambda v: 1 / mpmath.sqrt(v)),
