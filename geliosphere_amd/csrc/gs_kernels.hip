@@ -37,6 +37,62 @@ __device__ __forceinline__ void emit_event(const DevOut &O, uint64_t pid, uint32
     }
 }
 
+// Event staging of a plain run: each wave appends its records to its own
+// LDS area (slot from an LDS counter) and, at a cycle boundary once at least
+// kStageFlush are staged, writes them out together: the wave reserves a
+// contiguous range of the event region and its 64 lanes store consecutive
+// 8-byte words, so records reach HBM as full lines instead of four partial
+// 16-byte writes from one lane.  Records that find the area full are written
+// directly.
+constexpr int kStage = 32;         // records per wave
+constexpr int kStageFlush = 16;
+constexpr int kRecWords = (int)(sizeof(DevEvent) / sizeof(uint64_t));
+
+struct Stage {
+    uint64_t words[kBlockThreads / kWave][kStage * kRecWords];
+    uint32_t count[kBlockThreads / kWave];
+};
+
+__device__ __forceinline__ void stage_event(Stage *st, const DevOut &O, uint64_t pid, double t_inj, double t_exit,
+                                            double r, double theta, double sumac) {
+    const int w = (int)(threadIdx.x / kWave);
+    const uint32_t slot = atomicAdd(&st->count[w], 1u);
+    if (slot < (uint32_t)kStage) {
+        uint64_t *d = &st->words[w][slot * kRecWords];
+        d[0] = fm::double_to_bits(t_inj);
+        d[1] = fm::double_to_bits(t_exit);
+        d[2] = fm::double_to_bits(r);
+        d[3] = fm::double_to_bits(theta);
+        d[4] = fm::double_to_bits(sumac);
+        d[5] = pid;
+        d[6] = 0;
+        d[7] = 0;
+    } else {
+        emit_event(O, pid, 0, t_inj, t_exit, r, theta, sumac);
+    }
+}
+
+// Called by the whole wave (uniform control flow).
+__device__ __forceinline__ void stage_flush(Stage *st, const DevOut &O, uint32_t min_count) {
+    const int w = (int)(threadIdx.x / kWave), lane = (int)(threadIdx.x & (kWave - 1));
+    __builtin_amdgcn_wave_barrier();
+    uint32_t c = st->count[w];
+    if (c > (uint32_t)kStage) c = kStage;
+    if (c < min_count || c == 0) return;
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd((unsigned long long *)O.n_events, (unsigned long long)c);
+    base = __shfl(base, 0);
+    const uint64_t cap = O.ev_cap[0];
+    uint64_t *g = (uint64_t *)(O.events + O.ev_off[0]);
+    for (uint32_t i = (uint32_t)lane; i < c * kRecWords; i += kWave) {
+        const uint64_t rec = base + i / kRecWords;
+        if (rec < cap) __builtin_nontemporal_store(st->words[w][i], &g[base * kRecWords + i]);
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) st->count[w] = 0;
+    __builtin_amdgcn_wave_barrier();
+}
+
 // Per-simulation statistics a batch lane has gathered since it moved to its
 // current simulation.
 __device__ __forceinline__ void flush_sim_stats(const DevOut &O, uint32_t sim, uint64_t steps,
@@ -61,7 +117,7 @@ __device__ __forceinline__ void flush_sim_stats(const DevOut &O, uint32_t sim, u
 //   lane's simulation index never decreases.
 template <class Model, bool kBatch>
 __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, const uint64_t *off,
-                          uint64_t *stat, const DevOut &O) {
+                          uint64_t *stat, Stage *stage, const DevOut &O) {
     constexpr int NPS = Model::kNormals;   // normals per step; a cycle uses NPS Philox blocks
     unsigned long long *queue = (unsigned long long *)O.queue;
     const int lane = (int)(threadIdx.x & (kWave - 1));
@@ -88,7 +144,8 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
     uint32_t my_particles = 0, my_nan = 0, my_cap = 0, cycles = 0;
 
     auto emit = [&](double t_inj, double t_exit, double r, double theta, double sumac) {
-        emit_event(O, pid, sim, t_inj, t_exit, r, theta, sumac);
+        if constexpr (kBatch) emit_event(O, pid, sim, t_inj, t_exit, r, theta, sumac);
+        else stage_event(stage, O, pid, t_inj, t_exit, r, theta, sumac);
     };
     auto do_step = [&](const double *z) -> bool {
         if constexpr (kBatch) return m.step(lp, z, emit);
@@ -169,6 +226,7 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
                 }
             }
         }
+        if constexpr (!kBatch) stage_flush(stage, O, kStageFlush);
         if (__ballot(active) == 0) {
             if (exhausted) break;
             continue;
@@ -243,6 +301,7 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
         my_steps += steps;
         if (isnan(m.r)) ++my_nan;
     }
+    if constexpr (!kBatch) stage_flush(stage, O, 1);
     if constexpr (kBatch) {
         if (my_particles) flush_sim_stats(O, sim, my_steps, my_particles, my_nan, my_cap);
         if (lane == 0) atomicAdd((unsigned long long *)O.cycles, (unsigned long long)cycles);
@@ -294,9 +353,11 @@ template <class Model, int kMinWaves>
 __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(kMinWaves)))
 plain_kernel(const DevParams *__restrict__ Pp, const DevOut O) {
     __shared__ uint64_t stat[ST_COUNT_];
+    __shared__ Stage stage;
     if (threadIdx.x < ST_COUNT_) stat[threadIdx.x] = 0;
+    if (threadIdx.x < kBlockThreads / kWave) stage.count[threadIdx.x] = 0;
     __syncthreads();
-    run_lanes<Model, false>(Pp, 1, nullptr, stat, O);
+    run_lanes<Model, false>(Pp, 1, nullptr, stat, &stage, O);
     __syncthreads();
     if (threadIdx.x < ST_COUNT_ && stat[threadIdx.x])
         atomicAdd((unsigned long long *)&O.g_stats[threadIdx.x], (unsigned long long)stat[threadIdx.x]);
@@ -314,7 +375,7 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
     __shared__ uint64_t off[kBatchMaxSims + 1];
     for (uint32_t i = threadIdx.x; i <= n_sims; i += blockDim.x) off[i] = offsets[i];
     __syncthreads();
-    run_lanes<Model, true>(sims, n_sims, off, nullptr, O);
+    run_lanes<Model, true>(sims, n_sims, off, nullptr, nullptr, O);
 }
 
 // Histogram pass: workgroup (x, s) folds a strided share of simulation s's
