@@ -27,6 +27,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6          # MI355X dense FP64 vector
+# HBM bytes per particle of the 1D B-p trajectory kernel: (FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0) KiB per launch, scaled by a 256 MiB copy, over 1e8
+# particles (profiles/plain_bp_hbm_traffic.md).  Algorithmic: one 64-byte
+# event record per particle.
+BP_TRAFFIC_BYTES_PER_PARTICLE = 76.3
 # Algorithmic FLOPs per particle-step of the reference formulation, counted
 # by the oracle's op-counting scalar (tests/test_oracle.py::test_flop_count_per_step).
 ALG_FLOPS_PER_STEP = {0: 42.0, 1: 40.0, 2: 164.0, 3: 258.6}
@@ -221,7 +226,10 @@ def main():
                                "V=400 km/s, dt=500 s, T_inj log-uniform 0.01-100 GeV, 30 log bins",
                    "particles_per_model": n},
         "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": BP_TRAFFIC_BYTES_PER_PARTICLE * n if dom == 1 else None,
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
+                     "algorithmic_bytes": 64 * n,
                      "kernel": "plain_kernel<%s>" % kname, "kernel_ms": bp_ms,
                      "alg_flops_per_step": ALG_FLOPS_PER_STEP[dom],
                      "steps_per_launch": bp_steps},
