@@ -191,7 +191,8 @@ def main():
             q = abi.clone_params(p, seed=p.seed + 1000 * i)
             s, _, _ = eng.run_plain(q)
             st = eng.last_stats()
-            res.append((p.model, s.total_steps, st.kernel_ms, s))
+            util = st.active_cycles / st.lane_cycles if st.lane_cycles else 0.0
+            res.append((p.model, s.total_steps, st.kernel_ms, s, util))
         return res
 
     for i in range(a.warmup):
@@ -238,6 +239,8 @@ def main():
             "steps_per_launch": {abi.MODEL_NAMES[m]: sum(r[1] for r in runs if r[0] == m) / a.steps
                                  for m in models},
             "spectra_per_s": len(models) * a.steps / wall,
+            "lane_utilisation": {abi.MODEL_NAMES[m]: sum(r[4] for r in runs if r[0] == m) / a.steps
+                                 for m in models},
         },
     }
     if a.batch_sims > 0:

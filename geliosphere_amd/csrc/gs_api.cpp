@@ -55,6 +55,11 @@ struct gs_ctx {
     std::vector<uint64_t> graph_key;
     hipGraphExec_t graph_exec = nullptr;
     uint64_t batch_event_budget = 1ull << 29;
+    // plain runs: scheduling order of a chunk's particles (gs_set_order)
+    int32_t order_mode = GS_ORDER_ENERGY;
+    uint32_t *d_order = nullptr;        // permutation, cap_order entries
+    uint64_t cap_order = 0;
+    uint32_t *d_order_counts = nullptr;  // 2 * kOrderMaxBuckets words
     uint32_t block_override[4] = {0, 0, 0, 0};
     uint32_t bpcu_override[4] = {0, 0, 0, 0};
     gs_run_stats stats{};
@@ -173,6 +178,45 @@ gs_status ensure_events(gs_ctx *ctx, uint64_t n) {
     return GS_OK;
 }
 
+gs_status ensure_order(gs_ctx *ctx, uint64_t n) {
+    if (!ctx->d_order_counts)
+        HIP_TRY(ctx, hipMalloc((void **)&ctx->d_order_counts, 2 * gs::kOrderMaxBuckets * sizeof(uint32_t)));
+    if (ctx->cap_order >= n) return GS_OK;
+    if (ctx->d_order) { (void)hipFree(ctx->d_order); ctx->d_order = nullptr; ctx->cap_order = 0; }
+    HIP_TRY(ctx, hipMalloc((void **)&ctx->d_order, n * sizeof(uint32_t)));
+    ctx->cap_order = n;
+    return GS_OK;
+}
+
+// Energy buckets of the scheduling order (nb = 0: keep id order).  The
+// backward models' lifetimes fall steeply with injection energy; 1D F-p
+// particles are short-lived with no such trend, and a fixed injection
+// energy has nothing to sort.  Buckets: the top bits of T's double
+// representation (16 per binade and up), at most 512 over the run's T range.
+gs::OrderKey order_key(const gs_params &p, const DevParams &d) {
+    gs::OrderKey k{0, 0, 0};
+    if (p.model == GS_MODEL_1D_FP) return k;
+    double lo, hi;
+    switch (p.injection) {
+    case GS_INJ_LOG_UNIFORM: lo = p.T_min; hi = p.T_max; break;
+    case GS_INJ_GRID: lo = p.T_min; hi = p.T_min + (double)(p.grid_n ? p.grid_n - 1 : 0) * d.grid_step; break;
+    case GS_INJ_SOLARPROP_BINS: lo = 0.01; hi = 1278.34; break;
+    default: return k;
+    }
+    if (!(lo > 0.0) || !(hi > lo) || !std::isfinite(hi)) return k;
+    uint64_t blo, bhi;
+    std::memcpy(&blo, &lo, sizeof lo);
+    std::memcpy(&bhi, &hi, sizeof hi);
+    uint32_t shift = 40;
+    while ((bhi >> shift) - (blo >> shift) + 1 > 512) ++shift;
+    const uint64_t nb = (bhi >> shift) - (blo >> shift) + 1;
+    if (nb < 2) return k;
+    k.shift = shift;
+    k.kmin = blo >> shift;
+    k.nb = (uint32_t)nb;
+    return k;
+}
+
 // Layout of the word buffer for `sims` simulations.
 struct Layout {
     uint64_t *queue, *count, *digits, *stats;
@@ -268,6 +312,7 @@ unsigned fold_blocks(uint64_t events) {
 // Particles per trajectory launch: bounds the event buffer (B-p and 2D models
 // produce at most one event per particle) to 2^27 x 64 B = 8.6 GB.
 const uint64_t kMaxChunk = 1ull << 27;
+const uint64_t kOrderMinParticles = 4096;   // smaller runs keep id order
 
 uint64_t event_room(int model, uint64_t n) {
     if (model == GS_MODEL_1D_FP) return std::max<uint64_t>(n / 16, 1u << 16);
@@ -421,6 +466,7 @@ gs_status run_sub_batch(gs_ctx *ctx, const gs_params *sims, const std::vector<ui
             DevOut O;
             O.queue = ctx->d_meta + q_base + l;
             O.cycles = ctx->d_meta + c_base + l;
+            O.order = nullptr;
             O.g_count = W.count + (size_t)L.base * GS_MAX_BINS;
             O.g_digits = W.digits + (size_t)L.base * gs::kDigitsPerSim;
             O.g_stats = W.stats + (size_t)L.base * gs::ST_COUNT_;
@@ -438,7 +484,7 @@ gs_status run_sub_batch(gs_ctx *ctx, const gs_params *sims, const std::vector<ui
         }
         if (e == hipSuccess) {
             DevOut O;
-            O.queue = nullptr; O.cycles = nullptr;
+            O.queue = nullptr; O.cycles = nullptr; O.order = nullptr;
             O.g_count = W.count; O.g_digits = W.digits; O.g_stats = W.stats;
             O.events = ctx->d_events;
             O.ev_off = ctx->d_meta; O.ev_cap = ctx->d_meta + m; O.n_events = ctx->d_meta + 2 * m;
@@ -531,6 +577,8 @@ void gs_destroy(gs_ctx *ctx) {
     if (ctx->d_words) (void)hipFree(ctx->d_words);
     if (ctx->d_events) (void)hipFree(ctx->d_events);
     if (ctx->d_meta) (void)hipFree(ctx->d_meta);
+    if (ctx->d_order) (void)hipFree(ctx->d_order);
+    if (ctx->d_order_counts) (void)hipFree(ctx->d_order_counts);
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     for (int i = 0; i < 3; ++i) {
         if (ctx->side[i]) (void)hipStreamDestroy(ctx->side[i]);
@@ -587,6 +635,14 @@ gs_status gs_set_launch(gs_ctx *ctx, int32_t model, uint32_t block_threads, uint
     if (blocks_per_cu > 64) return fail(ctx, GS_ERR_INVALID_ARG, "blocks_per_cu above 64");
     ctx->block_override[model] = block_threads;
     ctx->bpcu_override[model] = blocks_per_cu;
+    return GS_OK;
+}
+
+gs_status gs_set_order(gs_ctx *ctx, int32_t mode) {
+    if (!ctx) return GS_ERR_INVALID_ARG;
+    if (mode != GS_ORDER_ID && mode != GS_ORDER_ENERGY)
+        return fail(ctx, GS_ERR_INVALID_ARG, "unknown order mode %d", (int)mode);
+    ctx->order_mode = mode;
     return GS_OK;
 }
 
@@ -652,6 +708,7 @@ gs_status gs_run_plain(gs_ctx *ctx, const gs_params *p, gs_spectrum *out, gs_exi
     O.queue = L.queue; O.g_count = L.count; O.g_digits = L.digits; O.g_stats = L.stats;
     O.ev_off = ctx->d_meta; O.ev_cap = ctx->d_meta + 1; O.n_events = ctx->d_meta + 2;
     O.cycles = ctx->d_meta + 3;
+    O.order = nullptr;
     ctx->h_meta.assign(2, 0);
     for (uint64_t c0 = 0; c0 < N || (N == 0 && c0 == 0); c0 += kMaxChunk) {
         gs_params q = *p;
@@ -662,6 +719,15 @@ gs_status gs_run_plain(gs_ctx *ctx, const gs_params *p, gs_spectrum *out, gs_exi
         if (st != GS_OK) return st;
         HIP_TRY(ctx, hipMemcpyAsync(ctx->d_params, ctx->h_params.data(), sizeof(DevParams),
                                     hipMemcpyHostToDevice, ctx->stream));
+        const gs::OrderKey key = order_key(q, ctx->h_params[0]);
+        O.order = nullptr;
+        if (ctx->order_mode == GS_ORDER_ENERGY && key.nb > 0 && q.n_particles >= kOrderMinParticles) {
+            st = ensure_order(ctx, q.n_particles);
+            if (st != GS_OK) return st;
+            HIP_TRY(ctx, gs_launch_order(ctx->d_params, q.n_particles, key, ctx->d_order_counts, ctx->d_order,
+                                         ctx->stream));
+            O.order = ctx->d_order;
+        }
         uint64_t nev = 0;
         for (int attempt = 0; attempt < 2; ++attempt) {
             ctx->h_meta[0] = 0;
@@ -754,6 +820,10 @@ gs_status gs_debug_normals(gs_ctx *ctx, uint64_t seed, uint64_t id, uint64_t n, 
     return GS_OK;
 }
 
+// The device kernels' Box-Muller tables, for the host evaluation.
+static const double kBmTrigHost[gs::fm::kBmTrigWords] = GS_BM_TRIG_TABLE;
+static const double kBmLogHost[gs::fm::kBmLogWords] = GS_BM_LOG_TABLE;
+
 gs_status gs_host_math(int32_t fn, const double *x, double *y, uint64_t n) {
     if (!x || !y) return GS_ERR_INVALID_ARG;
     namespace fm = gs::fm;
@@ -763,9 +833,9 @@ gs_status gs_host_math(int32_t fn, const double *x, double *y, uint64_t n) {
         switch (fn) {
         case GS_DBG_LOG: y[i] = fm::log_e<H>(x[i], 0); break;
         case GS_DBG_EXP: y[i] = fm::exp_e<H>(x[i]); break;
-        case GS_DBG_SIN_TURN: fm::sincos_turn<H>((uint32_t)x[i], s, c); y[i] = s; break;
-        case GS_DBG_COS_TURN: fm::sincos_turn<H>((uint32_t)x[i], s, c); y[i] = c; break;
-        case GS_DBG_LOG_U32: y[i] = fm::log_u32<H>((uint32_t)x[i]); break;
+        case GS_DBG_SIN_TURN: fm::sincos_turn<H>((uint32_t)x[i], kBmTrigHost, s, c); y[i] = s; break;
+        case GS_DBG_COS_TURN: fm::sincos_turn<H>((uint32_t)x[i], kBmTrigHost, s, c); y[i] = c; break;
+        case GS_DBG_LOG_U32: y[i] = fm::log_u32<H>((uint32_t)x[i], kBmLogHost); break;
         case GS_DBG_SIN: fm::sincos_small<H>(x[i], s, c); y[i] = s; break;
         case GS_DBG_COS: fm::sincos_small<H>(x[i], s, c); y[i] = c; break;
         case GS_DBG_ATAN: y[i] = fm::atan_e<H>(x[i]); break;

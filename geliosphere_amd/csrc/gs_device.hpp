@@ -2,15 +2,18 @@
 //
 // One lane integrates one particle at a time.  Per-particle state lives in
 // VGPRs; nothing is read from or written to HBM per step.  Events (exits,
-// 1 AU crossings) are folded into a workgroup-private LDS histogram whose
-// weighted sums are exact integers (fixed point, 32-bit digits in 64-bit
-// LDS words), so the result does not depend on the order in which lanes,
-// waves or workgroups fold their events.
+// 1 AU crossings) become 64-byte records in the simulation's event region;
+// the fold kernel bins them into histograms whose weighted sums are exact
+// integers (fixed point, 32-bit digits in 64-bit words), so the result does
+// not depend on the order in which lanes, waves or workgroups add events.
 //
 // Model semantics follow the reference CPU path (thresholds, reflections,
 // order of normal draws); the arithmetic is re-associated for the GPU
 // (shared reciprocals, CSE of sin/cos, pow(Cb, k/2) as products of sqrt(Cb),
-// tan(acos x) = sqrt(1 - x^2) / x).
+// tan(acos x) = sqrt(1 - x^2) / x).  sqrt_nn where the argument cannot be
+// negative (T > 0: energies only grow on the backward paths; 1 + gamma^2;
+// |x| < 1; diffusion coefficients); sqrt_pos for B11, whose determinant can
+// round below zero.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -39,6 +42,11 @@ constexpr double kBetaMinBp = (double)0.01f;
 constexpr double kReflectBp = (double)0.3f;
 
 // ------------------------------------------------------------------ RNG
+// a ^ b ^ c in one instruction (gfx950 v_bitop3_b32, truth table 0x96).
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
 __device__ __forceinline__ void philox10(uint32_t &x0, uint32_t &x1, uint32_t &x2, uint32_t &x3,
                                          uint32_t k0, uint32_t k1) {
@@ -46,8 +54,8 @@ __device__ __forceinline__ void philox10(uint32_t &x0, uint32_t &x1, uint32_t &x
     for (int round = 0; round < 10; ++round) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * x0;
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * x2;
-        const uint32_t y0 = (uint32_t)(p1 >> 32) ^ x1 ^ k0;
-        const uint32_t y2 = (uint32_t)(p0 >> 32) ^ x3 ^ k1;
+        const uint32_t y0 = xor3((uint32_t)(p1 >> 32), x1, k0);
+        const uint32_t y2 = xor3((uint32_t)(p0 >> 32), x3, k1);
         x0 = y0;
         x1 = (uint32_t)p1;
         x2 = y2;
@@ -57,21 +65,34 @@ __device__ __forceinline__ void philox10(uint32_t &x0, uint32_t &x1, uint32_t &x
     }
 }
 
+// Box-Muller tables (gs_bm_tables.h); every workgroup copies them into LDS
+// (load_bm_tables, then a barrier) and reads them from there.
+__constant__ double kBmTrigTable[fm::kBmTrigWords] = GS_BM_TRIG_TABLE;
+__constant__ double kBmLogTable[fm::kBmLogWords] = GS_BM_LOG_TABLE;
+struct BmTables {
+    double trig[fm::kBmTrigWords];
+    double log[fm::kBmLogWords];
+};
+__device__ __forceinline__ void load_bm_tables(BmTables *t) {
+    for (int i = (int)threadIdx.x; i < fm::kBmTrigWords; i += (int)blockDim.x) t->trig[i] = kBmTrigTable[i];
+    for (int i = (int)threadIdx.x; i < fm::kBmLogWords; i += (int)blockDim.x) t->log[i] = kBmLogTable[i];
+}
+
 // Normals of a particle: block k = Philox(ctr = {id lo, id hi, k, 0},
 // key = {seed lo, seed hi}); its words (x0,x1) and (x2,x3) give two
 // Box-Muller pairs, u1 = (x+1) 2^-32 in (0,1], u2 = x' 2^-32:
 // z = sqrt(-2 ln u1) (cos, sin)(2 pi u2).  Normal 4k+i is z[i].
 __device__ __forceinline__ void normal_block(uint64_t pid, uint32_t k, uint32_t k0, uint32_t k1,
-                                             double z[4]) {
+                                             const BmTables &bm, double z[4]) {
     uint32_t x0 = (uint32_t)pid, x1 = (uint32_t)(pid >> 32), x2 = k, x3 = 0u;
     philox10(x0, x1, x2, x3, k0, k1);
     double s, c;
-    const double ra = fm::sqrt_pos(-2.0 * fm::log_u32d(x0));
-    fm::sincos_turnd(x1, s, c);
+    const double ra = fm::sqrt_nn(-2.0 * fm::log_u32d(x0, bm.log));   // ln u1 <= 0
+    fm::sincos_turnd(x1, bm.trig, s, c);
     z[0] = ra * c;
     z[1] = ra * s;
-    const double rb = fm::sqrt_pos(-2.0 * fm::log_u32d(x2));
-    fm::sincos_turnd(x3, s, c);
+    const double rb = fm::sqrt_nn(-2.0 * fm::log_u32d(x2, bm.log));
+    fm::sincos_turnd(x3, bm.trig, s, c);
     z[2] = rb * c;
     z[3] = rb * s;
 }
@@ -259,13 +280,15 @@ __device__ void fold_one(const DevParams &P, uint64_t *lds, const DevEvent &e) {
 // 1D B-p.  State: r [AU], P = pc/q [GV], beta (of the last computed T, which
 // a reflection does not restore, as in OneDimensionBpCpuModel.cpp:108-140).
 // Pm.vdt = -V dt and Pm.cp = -(2/3) V dt carry the backward sign of V
-// (Constants.hpp:213-218).
+// (Constants.hpp:213-218).  T = sqrt(T0^2 + P^2) - T0 is formed only for an
+// exit; the test T < 200 is E^2 < (200 + T0)^2.
 struct Bp {
     static constexpr int kNormals = 1;
+    static constexpr double kE2Exit = (200.0 + kT0) * (200.0 + kT0);
     double r, P, beta, t_inj;
     __device__ __forceinline__ bool init(const StepParams &Pm, double T) {
         t_inj = T;
-        P = fm::sqrt_pos(T * (T + 2.0 * kT0));
+        P = fm::sqrt_nn(T * (T + 2.0 * kT0));
         beta = P * fm::rcp(T + kT0);
         r = Pm.r_inj;
         return r < 100.0002;
@@ -275,40 +298,34 @@ struct Bp {
         const double K2dt = Pm.k2dt * (beta * P);                     // 2 K dt
         const double inv_r = fm::rcp(r);
         const double Pn = fma(-(Pm.cp * P), inv_r, P);               // p - 2 V p dt / (3 r)
-        const double dr = fma(z[0], fm::sqrt_pos(K2dt), fma(K2dt, inv_r, Pm.vdt));
-        double rn = r + dr;
+        const double dr = fma(K2dt, inv_r, z[0] * fm::sqrt_nn(K2dt)) + Pm.vdt;
+        const double rn = r + dr;
+        const double rb = rn - dr;
         const double E2 = fma(Pn, Pn, kT0 * kT0);
         const double invE = fm::rsqrt(E2);
-        const double T = fma(E2, invE, -kT0);                         // sqrt(T0^2 + P^2) - T0
         const double bn = Pn * invE;
-        bool alive = true;
-        if (bn > kBetaMinBp && T < 200.0) {
-            if (rn > 100.0 && (rn - dr) < 100.0) {
-                emit(t_inj, T, rn, 0.0, 0.0);
-                r = rn;
-                return false;
-            }
-        } else if (bn < kBetaMinBp) {
-            alive = false;
-        }
+        const bool stop = bn < kBetaMinBp;
+        const bool out = bn > kBetaMinBp && E2 < kE2Exit && rn > 100.0 && rb < 100.0;
+        if (out) emit(t_inj, fma(E2, invE, -kT0), rn, 0.0, 0.0);
         beta = bn;
-        const bool refl = rn < kReflectBp;
-        rn = refl ? rn - dr : rn;     // a reflected step keeps P
+        const bool refl = rn < kReflectBp;   // a reflected step keeps P
+        r = refl ? rb : rn;
         P = refl ? P : Pn;
-        r = rn;
-        return alive && rn < 100.0002;
+        return !(out || stop) && r < 100.0002;
     }
 };
 
 // 1D F-p.  State: r, P [GV], beta and R (rigidity) of the last computed T,
 // sumac.  OneDimensionFpCpuModel.cpp:84-152.  Pm.vdt = V dt,
-// Pm.cp = (2/3) V dt, Pm.csum = (4/3) V dt.
+// Pm.cp = (2/3) V dt, Pm.csum = (4/3) V dt.  T is formed only for a crossing;
+// the test T < 100 is E^2 < (100 + T0)^2.
 struct Fp {
     static constexpr int kNormals = 1;
+    static constexpr double kE2Cross = (100.0 + kT0) * (100.0 + kT0);
     double r, P, beta, R, sumac, t_inj;
     __device__ __forceinline__ bool init(const StepParams &Pm, double T) {
         t_inj = T;
-        P = fm::sqrt_pos(T * (T + 2.0 * kT0));
+        P = fm::sqrt_nn(T * (T + 2.0 * kT0));
         R = P;
         beta = P * fm::rcp(T + kT0);
         sumac = 0.0;
@@ -319,17 +336,16 @@ struct Fp {
     __device__ __forceinline__ bool step(const StepParams &Pm, const double *z, const Emit &emit) {
         const double K2dt = Pm.k2dt * (beta * R);
         const double inv_r = fm::rcp(r);
-        const double dr = fma(z[0], fm::sqrt_pos(K2dt), fma(K2dt, inv_r, Pm.vdt));
+        const double dr = fma(K2dt, inv_r, z[0] * fm::sqrt_nn(K2dt)) + Pm.vdt;
         const double Pn = fma(-(Pm.cp * P), inv_r, P);
         sumac = fma(Pm.csum, inv_r, sumac);
         double rn = r + dr;
         const double E2 = fma(Pn, Pn, kT0 * kT0);
         const double invE = fm::rsqrt(E2);
-        const double T = fma(E2, invE, -kT0);
-        if (beta > 0.01 && T < 100.0) {
+        if (beta > 0.01 && E2 < kE2Cross) {
             // (r - 1) / ((r - dr) - 1) < 0, without the division
             const double a = rn - 1.0, b = (rn - dr) - 1.0;
-            if ((a < 0.0 && b >= 0.0) || (a > 0.0 && b < 0.0)) emit(t_inj, T, rn, 0.0, sumac);
+            if ((a < 0.0 && b >= 0.0) || (a > 0.0 && b < 0.0)) emit(t_inj, fma(E2, invE, -kT0), rn, 0.0, sumac);
         }
         const bool stop = beta < 0.01;
         beta = Pn * invE;
@@ -358,7 +374,7 @@ struct Sp {
     __device__ __forceinline__ bool step(const StepParams &Pm, const double *z, const Emit &emit) {
         double s, c;
         fm::sincosd(theta, s, c);
-        const double Rig = fm::sqrt_pos(T * (T + 2.0 * kT0));
+        const double Rig = fm::sqrt_nn(T * (T + 2.0 * kT0));
         const double inv_E = fm::rcp(T + kT0);
         const double beta = Rig * inv_E;
         const double r2 = r * r;
@@ -367,7 +383,7 @@ struct Sp {
         const double g2 = gamma * gamma;
         const double opg = 1.0 + g2;
         const double inv_opg = fm::rcp(opg);
-        const double sq = fm::sqrt_pos(opg);
+        const double sq = fm::sqrt_nn(opg);
         const double inv_sq = sq * inv_opg;
         const double Rc = (Rig < 0.1) ? 0.1 : Rig;
         const double Kpar = Pm.K0 * beta * Rc * (5.0 / 3.4) * r2 * inv_sq * (1.0 / 3.0);
@@ -381,17 +397,17 @@ struct Sp {
                           (inv_opg * inv_opg * inv_opg);
         const double dKrr = (t1 + t2) * (5.0 / (3.0 * 3.4));
         const double z1 = z[0];
-        double dr = ((-Pm.V) + (2.0 * Krr * inv_r) + dKrr) * Pm.dt + z1 * fm::sqrt_pos(Pm.two_dt * Krr);
+        double dr = ((-Pm.V) + (2.0 * Krr * inv_r) + dKrr) * Pm.dt + z1 * fm::sqrt_nn(Pm.two_dt * Krr);
         const double inv_s = fm::rcp(s);
         const double z2 = z[1];
-        double dth = Ktt * c * inv_s * inv_r * inv_r * Pm.dt * inv_opg + z2 * fm::sqrt_pos(Pm.two_dt * Ktt) * inv_r;
+        double dth = Ktt * c * inv_s * inv_r * inv_r * Pm.dt * inv_opg + z2 * fm::sqrt_nn(Pm.two_dt * Ktt) * inv_r;
         const double dTkin = -Pm.cp * ((T + 2.0 * kT0) * inv_E) * T * inv_r;
         // drifts
         const double Larmor = (0.0225 / kA) * Rig * r2 * inv_sq;
         const double sy = fm::sind(Pm.tilt + Larmor * (kPi / 90.0) * inv_r);
         const double x = sy * fm::rcp(kPi - sy);
         // the device rcp has no infinities: sqrt(1 - x^2) / 0 is set to +inf
-        const double tanH = x != 0.0 ? fm::sqrt_pos(1.0 - x * x) * fm::rcp(x) : INFINITY;
+        const double tanH = x != 0.0 ? fm::sqrt_nn(1.0 - x * x) * fm::rcp(x) : INFINITY;
         // acos x from tan(acos x) = sqrt(1 - x^2) / x
         const double aH = fm::atand(tanH);
         const double alphaH = x > 0.0 ? aH : (x < 0.0 ? kPi + aH : 0.5 * kPi);
@@ -440,7 +456,7 @@ struct Ge {
         constexpr double kf = 1.0 / (3.0 * (5.0 / 3.4));   // as written in GeliosphereCpuModel.cpp:210-299
         double s, c;
         fm::sincosd(theta, s, c);
-        const double Rig = fm::sqrt_pos(T * (T + 2.0 * kT0));
+        const double Rig = fm::sqrt_nn(T * (T + 2.0 * kT0));
         const double inv_E = fm::rcp(T + kT0);
         const double beta = Rig * inv_E;
         const double r2 = r * r;
@@ -455,7 +471,7 @@ struct Ge {
         const double g2 = gamma * gamma;
         const double Cb = 1.0 + g2 + r2 * deltarh2;
         const double inv_Cb = fm::rcp(Cb);
-        const double sqCb = fm::sqrt_pos(Cb);
+        const double sqCb = fm::sqrt_nn(Cb);
         const double inv_sqCb = sqCb * inv_Cb;
         const double inv_Cb15 = inv_Cb * inv_sqCb;
         const double inv_Cb25 = inv_Cb * inv_Cb15;
@@ -467,7 +483,7 @@ struct Ge {
         const double Ktt = Kper + r2 * deltarh2 * dK * inv_Cb;
         const double Krt = deltarh * dK * r * inv_Cb;
         const double inv_Ktt = fm::rcp(Ktt);
-        const double sq2Ktt = fm::sqrt_pos(2.0 * Ktt);
+        const double sq2Ktt = fm::sqrt_nn(2.0 * Ktt);
         const double B11 = fm::sqrt_pos(2.0 * (Krr * Ktt - Krt * Krt) * inv_Ktt);
         const double B12 = Krt * inv_Ktt * sq2Ktt;
         const double B22 = sq2Ktt * inv_r;
@@ -513,7 +529,7 @@ struct Ge {
         const double sy = fm::sind(Pm.tilt + Larmor * (kPi / 90.0) * inv_r);
         const double x = sy * fm::rcp(kPi - sy);
         // the device rcp has no infinities: sqrt(1 - x^2) / 0 is set to +inf
-        const double tanH = x != 0.0 ? fm::sqrt_pos(1.0 - x * x) * fm::rcp(x) : INFINITY;
+        const double tanH = x != 0.0 ? fm::sqrt_nn(1.0 - x * x) * fm::rcp(x) : INFINITY;
         // acos x from tan(acos x) = sqrt(1 - x^2) / x
         const double aH = fm::atand(tanH);
         const double alphaH = x > 0.0 ? aH : (x < 0.0 ? kPi + aH : 0.5 * kPi);

@@ -84,6 +84,16 @@ struct DevOut {
     const uint64_t *ev_cap;
     uint64_t *n_events;       // per simulation; may exceed ev_cap (then re-run with more room)
     uint64_t *cycles;         // wave cycles issued by the launch
+    const uint32_t *order;    // plain runs: queue position -> particle offset (nullptr: identity)
+};
+
+// Scheduling order of a plain run (gs_launch_order): particle offsets sorted
+// by a bucket of their injection energy, bucket = (bits(T) >> shift) - kmin
+// clamped to [0, nb); the bits of a positive double grow with its value.
+constexpr uint32_t kOrderMaxBuckets = 1024;
+struct OrderKey {
+    uint32_t shift, nb;
+    uint64_t kmin;
 };
 
 // LDS layout per workgroup: count[n_bins] | digits[n_weights][2][n_bins][16] | stat[ST_COUNT_]
@@ -105,6 +115,12 @@ hipError_t gs_traj_occupancy(int model, bool batch, int *blocks_per_cu);
 hipError_t gs_launch_traj(int model, bool batch, const gs::DevParams *sims, uint32_t n_sims,
                           const uint64_t *offsets, const gs::DevOut &O, unsigned grid,
                           hipStream_t s);
+// Scheduling order of the n = sims->n_particles particles of one plain
+// simulation: perm[0 .. n) receives the offsets 0 .. n-1 in ascending bucket
+// order (within a bucket in no fixed order).  counts: 2 * key.nb words of
+// scratch.
+hipError_t gs_launch_order(const gs::DevParams *sims, uint64_t n, const gs::OrderKey &key, uint32_t *counts,
+                           uint32_t *perm, hipStream_t s);
 // Histograms of n_sims simulations from their event regions; lds_bytes fits
 // the largest (n_bins, n_weights) among them.
 hipError_t gs_launch_fold(const gs::DevParams *sims, uint32_t n_sims, const gs::DevOut &O,

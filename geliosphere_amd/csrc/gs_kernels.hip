@@ -117,7 +117,7 @@ __device__ __forceinline__ void flush_sim_stats(const DevOut &O, uint32_t sim, u
 //   lane's simulation index never decreases.
 template <class Model, bool kBatch>
 __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, const uint64_t *off,
-                          uint64_t *stat, Stage *stage, const DevOut &O) {
+                          uint64_t *stat, Stage *stage, const BmTables &bm, const DevOut &O) {
     constexpr int NPS = Model::kNormals;   // normals per step; a cycle uses NPS Philox blocks
     unsigned long long *queue = (unsigned long long *)O.queue;
     const int lane = (int)(threadIdx.x & (kWave - 1));
@@ -219,7 +219,7 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
                     ++my_particles;
                     active = m.init(lp, T);
                 } else {
-                    pid = first + take;
+                    pid = first + (O.order ? (uint64_t)O.order[take] : take);
                     T = injection_energy(P0, pid);
                     ++my_particles;
                     active = m.init(P0, T);
@@ -237,7 +237,7 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
             // 1D: the four normals of the cycle come from one block; every
             // normal's position is known at compile time.
             double z[4];
-            normal_block(pid, blk, k0, k1, z);
+            normal_block(pid, blk, k0, k1, bm, z);
             if (__ballot(active && steps + kCycleSteps > cap) == 0) {
                 // no lane can reach its step cap in this cycle
 #pragma unroll
@@ -276,7 +276,7 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
                 for (int i = 0; i < NPS; ++i) {
                     if (used == 4) {
                         double t[4];
-                        normal_block(pid, nb, k0, k1, t);
+                        normal_block(pid, nb, k0, k1, bm, t);
                         b0 = t[0]; b1 = t[1]; b2 = t[2]; b3 = t[3];
                         ++nb;
                         used = 0;
@@ -354,10 +354,12 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
 plain_kernel(const DevParams *__restrict__ Pp, const DevOut O) {
     __shared__ uint64_t stat[ST_COUNT_];
     __shared__ Stage stage;
+    __shared__ BmTables bm;
     if (threadIdx.x < ST_COUNT_) stat[threadIdx.x] = 0;
     if (threadIdx.x < kBlockThreads / kWave) stage.count[threadIdx.x] = 0;
+    load_bm_tables(&bm);
     __syncthreads();
-    run_lanes<Model, false>(Pp, 1, nullptr, stat, &stage, O);
+    run_lanes<Model, false>(Pp, 1, nullptr, stat, &stage, bm, O);
     __syncthreads();
     if (threadIdx.x < ST_COUNT_ && stat[threadIdx.x])
         atomicAdd((unsigned long long *)&O.g_stats[threadIdx.x], (unsigned long long)stat[threadIdx.x]);
@@ -373,9 +375,11 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
                                                               const uint64_t *__restrict__ offsets,
                                                               const DevOut O) {
     __shared__ uint64_t off[kBatchMaxSims + 1];
+    __shared__ BmTables bm;
     for (uint32_t i = threadIdx.x; i <= n_sims; i += blockDim.x) off[i] = offsets[i];
+    load_bm_tables(&bm);
     __syncthreads();
-    run_lanes<Model, true>(sims, n_sims, off, nullptr, nullptr, O);
+    run_lanes<Model, true>(sims, n_sims, off, nullptr, nullptr, bm, O);
 }
 
 // Histogram pass: workgroup (x, s) folds a strided share of simulation s's
@@ -403,6 +407,9 @@ __global__ void __launch_bounds__(kBlockThreads) fold_kernel(const DevParams *__
 // normals of one particle.
 __global__ void debug_kernel(int fn, const double *x, double *y, uint64_t n, uint64_t seed,
                              uint64_t pid) {
+    __shared__ BmTables bm;
+    load_bm_tables(&bm);
+    __syncthreads();
     const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     double s, c, z[4];
@@ -412,9 +419,9 @@ __global__ void debug_kernel(int fn, const double *x, double *y, uint64_t n, uin
     case GS_DBG_SQRT: y[i] = fm::sqrt_pos(x[i]); break;
     case GS_DBG_LOG: y[i] = fm::log_fast(x[i]); break;
     case GS_DBG_EXP: y[i] = fm::exp_fast(x[i]); break;
-    case GS_DBG_SIN_TURN: fm::sincos_turnd((uint32_t)x[i], s, c); y[i] = s; break;
-    case GS_DBG_COS_TURN: fm::sincos_turnd((uint32_t)x[i], s, c); y[i] = c; break;
-    case GS_DBG_LOG_U32: y[i] = fm::log_u32d((uint32_t)x[i]); break;
+    case GS_DBG_SIN_TURN: fm::sincos_turnd((uint32_t)x[i], bm.trig, s, c); y[i] = s; break;
+    case GS_DBG_COS_TURN: fm::sincos_turnd((uint32_t)x[i], bm.trig, s, c); y[i] = c; break;
+    case GS_DBG_LOG_U32: y[i] = fm::log_u32d((uint32_t)x[i], bm.log); break;
     case GS_DBG_RAW_RCP: y[i] = __builtin_amdgcn_rcp(x[i]); break;
     case GS_DBG_RAW_RSQ: y[i] = __builtin_amdgcn_rsq(x[i]); break;
     case GS_DBG_RAW_SQRT: y[i] = __builtin_amdgcn_sqrt(x[i]); break;
@@ -422,10 +429,89 @@ __global__ void debug_kernel(int fn, const double *x, double *y, uint64_t n, uin
     case GS_DBG_COS: fm::sincosd(x[i], s, c); y[i] = c; break;
     case GS_DBG_ATAN: y[i] = fm::atand(x[i]); break;
     case GS_DBG_NORMALS:
-        normal_block(pid, (uint32_t)(i >> 2), (uint32_t)seed, (uint32_t)(seed >> 32), z);
+        normal_block(pid, (uint32_t)(i >> 2), (uint32_t)seed, (uint32_t)(seed >> 32), bm, z);
         y[i] = z[i & 3];
         break;
     default: y[i] = 0.0; break;
+    }
+}
+
+// ---------------------------------------------------------------- order
+// A plain run's particles sorted by injection energy.  On the backward paths
+// low-energy particles diffuse slowest and live longest (two orders of
+// magnitude across 0.01-100 GeV), so handing them out first keeps the run
+// from ending on a few long trajectories in otherwise idle waves.  Results do
+// not depend on the order: particles are keyed by id and sums are exact.
+__device__ __forceinline__ uint32_t order_bucket(const DevParams &P, uint64_t i, const OrderKey &key) {
+    const double T = injection_energy(P, P.first_particle + i);
+    const uint64_t k = (uint64_t)__double_as_longlong(T) >> key.shift;
+    const uint64_t b = k > key.kmin ? k - key.kmin : 0;
+    return b < key.nb ? (uint32_t)b : key.nb - 1;
+}
+
+constexpr int kOrderPerThread = 16;
+constexpr uint64_t kOrderTile = (uint64_t)kBlockThreads * kOrderPerThread;
+
+// counts[b] = particles in bucket b.
+__global__ void __launch_bounds__(kBlockThreads) order_count_kernel(const DevParams *__restrict__ sims, uint64_t n,
+                                                                   OrderKey key, uint32_t *counts) {
+    __shared__ uint32_t h[kOrderMaxBuckets];
+    for (uint32_t b = threadIdx.x; b < key.nb; b += blockDim.x) h[b] = 0;
+    __syncthreads();
+    const DevParams &P = sims[0];
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        atomicAdd(&h[order_bucket(P, i, key)], 1u);
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < key.nb; b += blockDim.x)
+        if (h[b]) atomicAdd(&counts[b], h[b]);
+}
+
+// cursor[b] = sum of counts[0 .. b) (one workgroup; nb <= kOrderMaxBuckets).
+__global__ void __launch_bounds__(kOrderMaxBuckets) order_scan_kernel(const uint32_t *counts, uint32_t *cursor,
+                                                                     uint32_t nb) {
+    __shared__ uint32_t v[kOrderMaxBuckets];
+    const uint32_t t = threadIdx.x;
+    v[t] = t < nb ? counts[t] : 0u;
+    __syncthreads();
+    for (uint32_t d = 1; d < kOrderMaxBuckets; d <<= 1) {
+        const uint32_t add = t >= d ? v[t - d] : 0u;
+        __syncthreads();
+        v[t] += add;
+        __syncthreads();
+    }
+    if (t < nb) cursor[t] = v[t] - counts[t];
+}
+
+// Workgroup x places the offsets of tile [x kOrderTile, (x+1) kOrderTile):
+// ranks within the tile from LDS counters, one range per bucket reserved
+// from the global cursors.
+__global__ void __launch_bounds__(kBlockThreads) order_scatter_kernel(const DevParams *__restrict__ sims, uint64_t n,
+                                                                     OrderKey key, uint32_t *cursor,
+                                                                     uint32_t *perm) {
+    __shared__ uint32_t h[kOrderMaxBuckets];
+    for (uint32_t b = threadIdx.x; b < key.nb; b += blockDim.x) h[b] = 0;
+    __syncthreads();
+    const DevParams &P = sims[0];
+    const uint64_t base = blockIdx.x * kOrderTile;
+    uint32_t bk[kOrderPerThread], rk[kOrderPerThread];
+#pragma unroll
+    for (int j = 0; j < kOrderPerThread; ++j) {
+        const uint64_t i = base + (uint64_t)j * blockDim.x + threadIdx.x;
+        bk[j] = 0;
+        rk[j] = 0;
+        if (i < n) {
+            bk[j] = order_bucket(P, i, key);
+            rk[j] = atomicAdd(&h[bk[j]], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < key.nb; b += blockDim.x)
+        h[b] = h[b] ? atomicAdd(&cursor[b], h[b]) : 0u;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kOrderPerThread; ++j) {
+        const uint64_t i = base + (uint64_t)j * blockDim.x + threadIdx.x;
+        if (i < n) perm[h[bk[j]] + rk[j]] = (uint32_t)i;
     }
 }
 
@@ -507,6 +593,22 @@ hipError_t gs_launch_traj(int model, bool batch, const DevParams *sims, uint32_t
     case GS_MODEL_2D_GELIOSPHERE: launch_traj_t<Ge, kBatchWavesGe, kPlainWavesGe>(batch, sims, n_sims, offsets, O, grid, s); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t gs_launch_order(const DevParams *sims, uint64_t n, const OrderKey &key, uint32_t *counts,
+                           uint32_t *perm, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (key.nb == 0 || key.nb > kOrderMaxBuckets || n > 0xffffffffull || ((uintptr_t)counts & 7u))
+        return hipErrorInvalidValue;
+    uint32_t *cursor = counts + key.nb;
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(kBlockThreads), 0, s, (uint64_t *)counts, (size_t)key.nb);
+    const uint64_t blocks = (n + kBlockThreads - 1) / kBlockThreads;
+    hipLaunchKernelGGL(order_count_kernel, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(kBlockThreads), 0, s,
+                       sims, n, key, counts);
+    hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(kOrderMaxBuckets), 0, s, counts, cursor, key.nb);
+    hipLaunchKernelGGL(order_scatter_kernel, dim3((unsigned)((n + kOrderTile - 1) / kOrderTile)), dim3(kBlockThreads),
+                       0, s, sims, n, key, cursor, perm);
     return hipGetLastError();
 }
 

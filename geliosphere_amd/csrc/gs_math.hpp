@@ -7,13 +7,20 @@
 //                     Goldschmidt steps;
 //   log               exponent split, s = (m-1)/(m+1), 2s + 2s w R(w);
 //   exp               k ln2 + r reduction (two-part ln2), Taylor to r^13;
-//   sincos_turn       sin/cos(2 pi x 2^-32) for a 32-bit integer x, with
-//                     exact integer quadrant reduction (Box-Muller angle);
 //   sincos_small      sin/cos of |x| < 2^20 (Cody-Waite pi/2 reduction);
 //   atan              three-point argument reduction + odd polynomial;
-//   log_u32           ln((x + 1) 2^-32), the Box-Muller radius argument.
-// R, S, C, A are Chebyshev fits made by gen_poly.py (worst relative error of
-// the function value 4.5e-18, 1.4e-17, 6.9e-19, 3.2e-19 before rounding).
+//   sincos_turn       sin/cos(2 pi x 2^-32) for a 32-bit integer x and
+//   log_u32           ln((x + 1) 2^-32): the Box-Muller angle and radius
+//                     argument, table-driven (P. T. P. Tang, "Table-driven
+//                     implementation of the logarithm function in IEEE
+//                     floating-point arithmetic", ACM TOMS 16(4), 1990: a
+//                     table entry next to the argument and a short
+//                     polynomial in the remainder; the same scheme for
+//                     sin/cos).  The tables are gs_bm_tables.h; the kernels
+//                     keep them in LDS.
+// R, S, C, A, L are Chebyshev fits made by gen_poly.py (worst relative error
+// of the function value 4.5e-18, 1.4e-17, 6.9e-19, 3.2e-19, 3.7e-17 before
+// rounding).
 //
 // Every multiply-add is an explicit fused multiply-add (Ops::fma3 / fmac: on
 // the device v_fma_f64, on the host std::fma).  Both are correctly rounded,
@@ -29,6 +36,8 @@
 
 #include <cmath>
 
+#include "gs_bm_tables.h"
+
 #pragma clang fp contract(off)
 
 namespace gs {
@@ -37,7 +46,7 @@ namespace fm {
 constexpr double kLn2Hi = 6.93147180369123816490e-01;  // upper 32 bits of ln 2
 constexpr double kLn2Lo = 1.90821492927058770002e-10;  // ln 2 - kLn2Hi
 constexpr double kInvLn2 = 1.44269504088896338700e+00;
-constexpr double kHalfPiOver2p30 = 1.5707963267948966 / 1073741824.0;  // (pi/2) 2^-30
+constexpr double kTwoPiOver2p32 = 1.4629180792671596e-09;   // 2 pi 2^-32
 
 __host__ __device__ inline double bits_to_double(uint64_t b) {
     union { uint64_t u; double d; } v;
@@ -70,20 +79,24 @@ __device__ __forceinline__ double rsqrt(double x) {
     return fma(y, e, y);
 }
 
-// sqrt(x) for x >= 0 (0 -> 0).  Goldschmidt: from the seed y ~ 1/sqrt(x),
-// g = x y -> sqrt(x) and h = y/2 -> 1/(2 sqrt(x)) converge together (one
-// step squares the seed's ~2^-24 error); a final residual step
-// g += (x - g^2) h rounds g to within an ulp.
-__device__ __forceinline__ double sqrt_pos(double x) {
-    const double y = __builtin_amdgcn_rsq(x);
+// sqrt(x) for x >= 0 (0 -> 0; negative x: undefined).  Goldschmidt: from
+// the seed y ~ 1/sqrt(x), g = x y -> sqrt(x) and h = y/2 -> 1/(2 sqrt(x))
+// converge together (one step squares the seed's ~2^-24 error); a final
+// residual step g += (x - g^2) h rounds g to within an ulp.  The seed is
+// taken at max(x, DBL_MIN), so x = 0 runs through with g = 0 and h finite:
+// no select.
+__device__ __forceinline__ double sqrt_nn(double x) {
+    const double y = __builtin_amdgcn_rsq(fmax(x, 2.2250738585072014e-308));
     double g = x * y, h = 0.5 * y;
     const double r = fma(-g, h, 0.5);
     g = fma(g, r, g);
     h = fma(h, r, h);
     const double d = fma(-g, g, x);
-    g = fma(d, h, g);
-    return x > 0.0 ? g : 0.0;
+    return fma(d, h, g);
 }
+
+// sqrt(x), 0 for x <= 0.
+__device__ __forceinline__ double sqrt_pos(double x) { return x > 0.0 ? sqrt_nn(x) : 0.0; }
 
 // Ops policy: fma3 (general), fmac (addend a constant: Horner steps),
 // inv (reciprocal).
@@ -174,22 +187,6 @@ __host__ __device__ inline void sincos_poly(double a, double &s, double &c) {
     c = Ops::fma3(w * w, pc, Ops::fma3(-0.5, w, 1.0));
 }
 
-// sin/cos(2 pi x 2^-32): quadrant k = round(x / 2^30) mod 4, remainder
-// d = x - k 2^30 in [-2^29, 2^29) exactly, angle a = d (pi/2) 2^-30.
-template <class Ops>
-__host__ __device__ inline void sincos_turn(uint32_t x, double &s, double &c) {
-    const uint32_t k = ((x + 0x20000000u) >> 30) & 3u;
-    const int32_t d = (int32_t)(x - (k << 30));
-    const double a = (double)d * kHalfPiOver2p30;
-    double sa, ca;
-    sincos_poly<Ops>(a, sa, ca);
-    // sin/cos(k pi/2 + a)
-    const double ss = (k & 1u) ? ca : sa;
-    const double cc = (k & 1u) ? sa : ca;
-    s = (k & 2u) ? -ss : ss;
-    c = ((k + 1u) & 2u) ? -cc : cc;
-}
-
 // sin and cos of x for |x| < 2^20: k = rint(x 2/pi), a = x - k pi/2 with
 // pi/2 in two parts (the first has 31 fraction bits, so k * kPio2Hi is
 // exact), |a| <= pi/4, then the quadrant k mod 4.
@@ -239,18 +236,61 @@ __host__ __device__ inline double atan_e(double x) {
     return x < 0.0 ? -r : r;
 }
 
-// ln((x + 1) 2^-32) for a 32-bit integer x, in [-22.18, 0].
+// ------------------------------------------------------------ Box-Muller
+constexpr int kBmTrigWords = 512;   // (sin, cos)(2 pi j/256), j = 0..255
+constexpr int kBmLogWords = 512;    // 128 x (invc, -ln invc hi, lo, 0)
+constexpr uint64_t kLogOff = 0x3fe5f00000000000ull;   // bits of 0.685546875
+
+// sin/cos(2 pi x 2^-32): j = round(x 2^-24) mod 256, d = x - j 2^24 in
+// [-2^23, 2^23) exactly, a = d 2 pi 2^-32 (|a| <= pi/256, w = a^2);
+// sin a = a + a w (-1/6 + w/120), cos a - 1 = w (-1/2 + w/24 - w^2/720)
+// (next terms below 1e-17), then sin(A + a) = sA + (sA (cos a - 1) + cA sin a)
+// and cos(A + a) = cA + (cA (cos a - 1) - sA sin a) with sA, cA from `trig`.
 template <class Ops>
-__host__ __device__ inline double log_u32(uint32_t x) {
-    return log_e<Ops>((double)x + 1.0, -32);   // (double)x + 1 is exact
+__host__ __device__ inline void sincos_turn(uint32_t x, const double *trig, double &s, double &c) {
+    const uint32_t j = (x + 0x800000u) >> 24;
+    const int32_t d = (int32_t)(x - (j << 24));
+    const double a = (double)d * kTwoPiOver2p32;
+    const double w = a * a;
+    const double sa = Ops::fma3(a * w, Ops::fmac(1.0 / 120.0, w, -1.0 / 6.0), a);
+    const double cm = w * Ops::fmac(Ops::fmac(-1.0 / 720.0, w, 1.0 / 24.0), w, -0.5);
+    const double sA = trig[2 * j], cA = trig[2 * j + 1];
+    s = Ops::fma3(cA, sa, Ops::fma3(sA, cm, sA));
+    c = Ops::fma3(-sA, sa, Ops::fma3(cA, cm, cA));
+}
+
+// ln((x + 1) 2^-32) for a 32-bit integer x, in [-22.18, 0].  v = x + 1
+// (exact) = 2^e z: tmp = bits(v) - kLogOff, e = tmp >> 52 (0 .. 32), z =
+// bits(v) - e 2^52 in [0.6855, 1.3711), interval i = bits 45..51 of tmp.
+// ln v = e ln2 + ln(1/invc_i) + ln(1 + r) with r = z invc_i - 1, |r| <= 2^-8,
+// and ln(1 + r) = r + r^2 L(r).  The interval around 1 has invc = 1 and
+// ln(1/invc) = 0, so near v = 2^32 (e = 32) the result is ln(1 + r) with
+// r = z - 1 exact: relative accuracy where the result is tiny.
+template <class Ops>
+__host__ __device__ inline double log_u32(uint32_t x, const double *logt) {
+    const uint64_t ix = double_to_bits((double)x + 1.0);   // (double)x + 1 is exact
+    const uint64_t tmp = ix - kLogOff;
+    const int e = (int)((int64_t)tmp >> 52);
+    const uint32_t i = (uint32_t)(tmp >> 45) & 127u;
+    const double z = bits_to_double(ix - ((uint64_t)e << 52));
+    const double invc = logt[4 * i], lc_hi = logt[4 * i + 1], lc_lo = logt[4 * i + 2];
+    const double r = Ops::fma3(z, invc, -1.0);
+    double p = -0.1666691666986670666717;
+    p = Ops::fmac(p, r, 0.2000028571784131347818);
+    p = Ops::fmac(p, r, -0.2499999999899998399979);
+    p = Ops::fmac(p, r, 0.3333333333219045841247);
+    p = Ops::fmac(p, r, -0.5);
+    const double lp = Ops::fma3(r * r, p, r);
+    const double fe = (double)(e - 32);
+    return Ops::fmac(fe, kLn2Hi, lc_hi) + (lp + Ops::fmac(fe, kLn2Lo, lc_lo));
 }
 
 // The device forms used by the kernels.
 __device__ __forceinline__ double log_fast(double x) { return log_e<DevOps>(x, 0); }
 __device__ __forceinline__ double exp_fast(double x) { return exp_e<DevOps>(x); }
-__device__ __forceinline__ double log_u32d(uint32_t x) { return log_u32<DevOps>(x); }
-__device__ __forceinline__ void sincos_turnd(uint32_t x, double &s, double &c) {
-    sincos_turn<DevOps>(x, s, c);
+__device__ __forceinline__ double log_u32d(uint32_t x, const double *logt) { return log_u32<DevOps>(x, logt); }
+__device__ __forceinline__ void sincos_turnd(uint32_t x, const double *trig, double &s, double &c) {
+    sincos_turn<DevOps>(x, trig, s, c);
 }
 __device__ __forceinline__ void sincosd(double x, double &s, double &c) { sincos_small<DevOps>(x, s, c); }
 __device__ __forceinline__ double sind(double x) {

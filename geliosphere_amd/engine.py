@@ -15,8 +15,10 @@ _lib = None
 
 EXPORTS = ("gs_abi_version", "gs_create", "gs_destroy", "gs_last_error", "gs_default_params",
            "gs_validate_params", "gs_run_plain", "gs_run_batch", "gs_set_batch_budget",
-           "gs_set_launch", "gs_autotune", "gs_last_run_stats", "gs_debug_math", "gs_debug_normals",
+           "gs_set_launch", "gs_set_order", "gs_autotune", "gs_last_run_stats", "gs_debug_math", "gs_debug_normals",
            "gs_host_math")
+
+ORDER_ID, ORDER_ENERGY = 0, 1
 
 DBG_RCP, DBG_RSQRT, DBG_SQRT, DBG_LOG, DBG_EXP, DBG_SIN_TURN, DBG_COS_TURN, DBG_LOG_U32, \
     DBG_RAW_RCP, DBG_RAW_RSQ, DBG_RAW_SQRT, DBG_SIN, DBG_COS, DBG_ATAN = range(14)
@@ -57,6 +59,8 @@ def lib():
         L.gs_set_batch_budget.restype = ctypes.c_int
         L.gs_set_launch.argtypes = [vp, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32]
         L.gs_set_launch.restype = ctypes.c_int
+        L.gs_set_order.argtypes = [vp, ctypes.c_int32]
+        L.gs_set_order.restype = ctypes.c_int
         L.gs_autotune.argtypes = [vp, P(abi.GsParams), ctypes.c_uint32, P(ctypes.c_uint32),
                                   P(ctypes.c_double)]
         L.gs_autotune.restype = ctypes.c_int
@@ -167,6 +171,11 @@ class Engine:
 
     def set_launch(self, model, block_threads=0, blocks_per_cu=0):
         self._check(lib().gs_set_launch(self._ctx, model, block_threads, blocks_per_cu), "gs_set_launch")
+
+    def set_order(self, mode):
+        """ORDER_ENERGY (default) or ORDER_ID: the order in which plain runs
+        hand particles to the lanes; results do not depend on it."""
+        self._check(lib().gs_set_order(self._ctx, mode), "gs_set_order")
 
     def autotune(self, pilot, repeats=1):
         """Time the pilot at every blocks-per-CU setting; apply and return

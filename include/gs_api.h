@@ -210,6 +210,19 @@ gs_status gs_set_batch_budget(gs_ctx *ctx, uint64_t max_events);
 /* Override the launch shape used for `model` (0 = automatic). */
 gs_status gs_set_launch(gs_ctx *ctx, int32_t model, uint32_t block_threads,
                         uint32_t blocks_per_cu);
+
+/* Order in which gs_run_plain hands particles to the lanes (results do not
+ * depend on it: particles are keyed by id and sums are exact).
+ * GS_ORDER_ENERGY (default): ascending injection energy for the backward
+ * models, whose lifetimes fall steeply with energy, so the longest
+ * trajectories start first and the run does not end with most lanes idle;
+ * 1D F-p and fixed-energy runs keep id order.  GS_ORDER_ID: id order.
+ * Replaces the reference's fixed id-to-thread mapping
+ * (CUDAKernel/src/OneDimensionBpModel.cu:176-184: launch k integrates ids
+ * [k 2^20, (k+1) 2^20) and waits for its slowest particle). */
+#define GS_ORDER_ID 0
+#define GS_ORDER_ENERGY 1
+gs_status gs_set_order(gs_ctx *ctx, int32_t mode);
 /* Pick blocks per CU for `model` by timing the pilot simulation at every
  * occupancy from 1 to the kernel's maximum (best of `repeats` runs each) and
  * apply it as gs_set_launch would.  best_bpcu / best_steps_per_s may be NULL.

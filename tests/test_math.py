@@ -25,8 +25,15 @@ def samples(seed=1):
     log_x = [rng.uniform(0.5, 2.0) for _ in range(300)] + \
             [10.0 ** rng.uniform(-300, 300) for _ in range(700)] + [1.0, 2.0, math.sqrt(2), 1e-300]
     exp_x = [rng.uniform(-700, 700) for _ in range(600)] + [rng.uniform(-1, 1) for _ in range(400)] + [0.0]
+    # random words; the top of the range, where ln u1 is tiny and must stay
+    # accurate relative to itself; both sides of the sin/cos table's
+    # interval edges (j 2^24 +- 2^23) and of powers of two
     u32 = [float(rng.getrandbits(32)) for _ in range(1000)] + \
-          [0.0, 1.0, 2.0 ** 29, 2.0 ** 29 - 1, 2.0 ** 30, 2.0 ** 31, 3 * 2.0 ** 30, 2.0 ** 32 - 1]
+          [0.0, 1.0, 2.0 ** 29, 2.0 ** 29 - 1, 2.0 ** 30, 2.0 ** 31, 3 * 2.0 ** 30, 2.0 ** 32 - 1] + \
+          [float(2 ** 32 - k) for k in range(2, 200)] + \
+          [float((j << 24) + d) for j in range(0, 256, 5) for d in (-(1 << 23) - 1, -(1 << 23), (1 << 23) - 1)
+           if 0 <= (j << 24) + d < 2 ** 32] + \
+          [float((1 << k) + d) for k in range(1, 32) for d in (-2, -1, 0)]
     trig = [rng.uniform(-10, 10) for _ in range(400)] + [rng.uniform(0, math.pi) for _ in range(400)] + \
            [0.0, math.pi / 4, math.pi / 2, math.pi, 2 * math.pi, 1e-10, -3.0, 1000.5]
     at = [s * 10.0 ** rng.uniform(-8, 8) for s in (1, -1) for _ in range(400)] + \
