@@ -790,15 +790,17 @@ gs_status gs_run_plain(gs_ctx *ctx, const gs_params *p, gs_spectrum *out, gs_exi
 
 gs_status gs_debug_math(gs_ctx *ctx, int32_t fn, const double *x, double *y, uint64_t n) {
     if (!ctx || (!x && fn != GS_DBG_NORMALS) || !y) return GS_ERR_INVALID_ARG;
-    if (fn < GS_DBG_RCP || fn > GS_DBG_ATAN) return fail(ctx, GS_ERR_INVALID_ARG, "unknown fn");
+    if (fn < GS_DBG_RCP || fn > GS_DBG_SHEET_G) return fail(ctx, GS_ERR_INVALID_ARG, "unknown fn");
     if (!x) return GS_ERR_INVALID_ARG;
     if (n == 0) return GS_OK;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const bool two = fn == GS_DBG_ATAN_FRAC || fn == GS_DBG_SHEET_F || fn == GS_DBG_SHEET_G;
+    const uint64_t nin = two ? 2 * n : n;   // two-argument forms: 2n inputs
     double *d = nullptr;
-    HIP_TRY(ctx, hipMalloc((void **)&d, 2 * n * sizeof(double)));
-    hipError_t e = hipMemcpyAsync(d, x, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = gs_launch_debug(fn, d, d + n, n, 0, 0, ctx->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(y, d + n, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    HIP_TRY(ctx, hipMalloc((void **)&d, (nin + n) * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(d, x, nin * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = gs_launch_debug(fn, d, d + nin, n, 0, 0, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(y, d + nin, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(d);
     if (e != hipSuccess) return fail(ctx, GS_ERR_HIP, "gs_debug_math: %s", hipGetErrorString(e));
@@ -839,6 +841,9 @@ gs_status gs_host_math(int32_t fn, const double *x, double *y, uint64_t n) {
         case GS_DBG_SIN: fm::sincos_small<H>(x[i], s, c); y[i] = s; break;
         case GS_DBG_COS: fm::sincos_small<H>(x[i], s, c); y[i] = c; break;
         case GS_DBG_ATAN: y[i] = fm::atan_e<H>(x[i]); break;
+        case GS_DBG_ATAN_FRAC: y[i] = fm::atan_frac<H>(x[i], x[n + i]); break;
+        case GS_DBG_SHEET_F: fm::sheet_drift<H>(x[i], x[n + i], s, c); y[i] = s; break;
+        case GS_DBG_SHEET_G: fm::sheet_drift<H>(x[i], x[n + i], s, c); y[i] = c; break;
         default: return GS_ERR_UNSUPPORTED;   // rcp / rsqrt / sqrt seeds exist on the device only
         }
     }

@@ -10,10 +10,10 @@
 // Model semantics follow the reference CPU path (thresholds, reflections,
 // order of normal draws); the arithmetic is re-associated for the GPU
 // (shared reciprocals, CSE of sin/cos, pow(Cb, k/2) as products of sqrt(Cb),
-// tan(acos x) = sqrt(1 - x^2) / x).  sqrt_nn where the argument cannot be
-// negative (T > 0: energies only grow on the backward paths; 1 + gamma^2;
-// |x| < 1; diffusion coefficients); sqrt_pos for B11, whose determinant can
-// round below zero.
+// the neutral-sheet drift's acos/tan/atan chain in fraction form).  sqrt_nn
+// where the argument cannot be negative (T > 0: energies only grow on the
+// backward paths; 1 + gamma^2; pi (pi - 2 sin); diffusion coefficients);
+// sqrt_pos for B11, whose determinant can round below zero.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -358,6 +358,7 @@ struct Fp {
     }
 };
 
+
 // 2D SolarProp-like: SolarPropLikeCpuModel.cpp:93-256 (Kappl 2016, Eq. 13-18,
 // 26, 32-37, 42-44).
 struct Sp {
@@ -375,14 +376,13 @@ struct Sp {
         double s, c;
         fm::sincosd(theta, s, c);
         const double Rig = fm::sqrt_nn(T * (T + 2.0 * kT0));
-        const double inv_E = fm::rcp(T + kT0);
-        const double beta = Rig * inv_E;
         const double r2 = r * r;
-        const double inv_r = fm::rcp(r);
         const double gamma = r * Pm.omega_v * s;
         const double g2 = gamma * gamma;
         const double opg = 1.0 + g2;
-        const double inv_opg = fm::rcp(opg);
+        double inv_E, inv_r, inv_opg, inv_s;
+        fm::rcp4(T + kT0, r, opg, s, inv_E, inv_r, inv_opg, inv_s);
+        const double beta = Rig * inv_E;
         const double sq = fm::sqrt_nn(opg);
         const double inv_sq = sq * inv_opg;
         const double Rc = (Rig < 0.1) ? 0.1 : Rig;
@@ -398,26 +398,18 @@ struct Sp {
         const double dKrr = (t1 + t2) * (5.0 / (3.0 * 3.4));
         const double z1 = z[0];
         double dr = ((-Pm.V) + (2.0 * Krr * inv_r) + dKrr) * Pm.dt + z1 * fm::sqrt_nn(Pm.two_dt * Krr);
-        const double inv_s = fm::rcp(s);
         const double z2 = z[1];
         double dth = Ktt * c * inv_s * inv_r * inv_r * Pm.dt * inv_opg + z2 * fm::sqrt_nn(Pm.two_dt * Ktt) * inv_r;
         const double dTkin = -Pm.cp * ((T + 2.0 * kT0) * inv_E) * T * inv_r;
         // drifts
         const double Larmor = (0.0225 / kA) * Rig * r2 * inv_sq;
         const double sy = fm::sind(Pm.tilt + Larmor * (kPi / 90.0) * inv_r);
-        const double x = sy * fm::rcp(kPi - sy);
-        // the device rcp has no infinities: sqrt(1 - x^2) / 0 is set to +inf
-        const double tanH = x != 0.0 ? fm::sqrt_nn(1.0 - x * x) * fm::rcp(x) : INFINITY;
-        // acos x from tan(acos x) = sqrt(1 - x^2) / x
-        const double aH = fm::atand(tanH);
-        const double alphaH = x > 0.0 ? aH : (x < 0.0 ? kPi + aH : 0.5 * kPi);
-        const double arg = (1.0 - theta * (2.0 / kPi)) * tanH;
-        const double inv_aH = fm::rcp(alphaH);
-        const double f = fm::atand(arg) * inv_aH;
+        double f, g;
+        fm::sheet_drift<fm::DevOps>(sy, 1.0 - theta * (2.0 / kPi), f, g);
         const double dc = kKonvF * (2.0 / (3.0 * kA)) * Rig * beta * r * gamma * f * inv_opg * inv_opg;
         const double DriftR = Pm.polarity * dc * c * inv_s;
         const double DriftTheta = -Pm.pol_theta * dc * (2.0 + g2);
-        const double fprime = -(2.0 / kPi) * tanH * inv_aH * fm::rcp(1.0 + arg * arg);
+        const double fprime = -(2.0 / kPi) * g;
         const double DriftSheetR = Pm.polarity * kKonvF * (1.0 / (3.0 * kA)) * Rig * beta * r * gamma * fprime * inv_opg;
         dr = dr + (DriftR + DriftSheetR) * Pm.dt;
         dth = dth + DriftTheta * Pm.dt * inv_r;
@@ -457,11 +449,18 @@ struct Ge {
         double s, c;
         fm::sincosd(theta, s, c);
         const double Rig = fm::sqrt_nn(T * (T + 2.0 * kT0));
-        const double inv_E = fm::rcp(T + kT0);
-        const double beta = Rig * inv_E;
         const double r2 = r * r;
-        const double inv_r = fm::rcp(r);
-        const double inv_s = fm::rcp(s);
+        double inv_E, inv_r, inv_s, inv_E_r;
+        {
+            // 1/(T + T0), 1/r, 1/s from one reciprocal
+            const double Er = (T + kT0) * r;
+            const double R = fm::rcp(Er * s);
+            inv_s = Er * R;
+            inv_E_r = s * R;       // 1/((T + T0) r)
+            inv_E = r * inv_E_r;
+            inv_r = (T + kT0) * inv_E_r;
+        }
+        const double beta = Rig * inv_E;
         const bool polar = (theta < Pm.th_lo) || (theta > Pm.th_hi);
         const bool band = (theta > Pm.th_lo) && (theta < Pm.th_hi);
         const double delta = polar ? 0.003 : Pm.delta0 * inv_s;
@@ -527,19 +526,12 @@ struct Ge {
         // drifts (Kappl 2016 Eq. 26, 32-37 with Cb in place of 1 + gamma^2)
         const double Larmor = (0.0225 / kA) * Rig * r2 * inv_sqCb;
         const double sy = fm::sind(Pm.tilt + Larmor * (kPi / 90.0) * inv_r);
-        const double x = sy * fm::rcp(kPi - sy);
-        // the device rcp has no infinities: sqrt(1 - x^2) / 0 is set to +inf
-        const double tanH = x != 0.0 ? fm::sqrt_nn(1.0 - x * x) * fm::rcp(x) : INFINITY;
-        // acos x from tan(acos x) = sqrt(1 - x^2) / x
-        const double aH = fm::atand(tanH);
-        const double alphaH = x > 0.0 ? aH : (x < 0.0 ? kPi + aH : 0.5 * kPi);
-        const double arg = (1.0 - theta * (2.0 / kPi)) * tanH;
-        const double inv_aH = fm::rcp(alphaH);
-        const double f = fm::atand(arg) * inv_aH;
+        double f, g;
+        fm::sheet_drift<fm::DevOps>(sy, 1.0 - theta * (2.0 / kPi), f, g);
         const double dc = kKonvF * (2.0 / (3.0 * kA)) * Rig * beta * r * gamma * f * inv_Cb * inv_Cb;
         const double DriftR = Pm.polarity * dc * c * inv_s;
         const double DriftTheta = -Pm.polarity * dc * (2.0 + g2);
-        const double fprime = -(2.0 / kPi) * tanH * inv_aH * fm::rcp(1.0 + arg * arg);
+        const double fprime = -(2.0 / kPi) * g;
         const double DriftSheetR = Pm.polarity * kKonvF * (1.0 / (3.0 * kA)) * Rig * beta * r * gamma * fprime * inv_Cb;
         dr = dr + (DriftR + DriftSheetR) * Pm.dt;
         dth = dth + DriftTheta * Pm.dt * inv_r;

@@ -428,6 +428,9 @@ __global__ void debug_kernel(int fn, const double *x, double *y, uint64_t n, uin
     case GS_DBG_SIN: fm::sincosd(x[i], s, c); y[i] = s; break;
     case GS_DBG_COS: fm::sincosd(x[i], s, c); y[i] = c; break;
     case GS_DBG_ATAN: y[i] = fm::atand(x[i]); break;
+    case GS_DBG_ATAN_FRAC: y[i] = fm::atan_fracd(x[i], x[n + i]); break;
+    case GS_DBG_SHEET_F: fm::sheet_drift<fm::DevOps>(x[i], x[n + i], s, c); y[i] = s; break;
+    case GS_DBG_SHEET_G: fm::sheet_drift<fm::DevOps>(x[i], x[n + i], s, c); y[i] = c; break;
     case GS_DBG_NORMALS:
         normal_block(pid, (uint32_t)(i >> 2), (uint32_t)seed, (uint32_t)(seed >> 32), bm, z);
         y[i] = z[i & 3];
@@ -526,6 +529,9 @@ __global__ void zero_kernel(uint64_t *p, size_t n) {
 // ---------------------------------------------------------------- launchers
 using namespace gs;
 
+// Waves per SIMD asked of the register allocator, per kernel (1: no request;
+// the 1D kernels reach 4 waves unasked, SolarProp 3).  Measured: SolarProp at
+// 4 waves (128 VGPRs) spills and runs 5 % slower.
 #ifndef GS_BATCH_WAVES_GE
 #define GS_BATCH_WAVES_GE 2
 #endif
@@ -533,10 +539,13 @@ using namespace gs;
 #define GS_PLAIN_WAVES_1D 1
 #endif
 #ifndef GS_PLAIN_WAVES_GE
-#define GS_PLAIN_WAVES_GE 1
+#define GS_PLAIN_WAVES_GE 3   // 168 VGPRs, a few spills: +7 % over 2 waves at 186 VGPRs
+#endif
+#ifndef GS_PLAIN_WAVES_SP
+#define GS_PLAIN_WAVES_SP 1
 #endif
 constexpr int kBatchWaves1D = 1, kBatchWavesSp = 2, kBatchWavesGe = GS_BATCH_WAVES_GE;
-constexpr int kPlainWaves1D = GS_PLAIN_WAVES_1D, kPlainWavesSp = 1, kPlainWavesGe = GS_PLAIN_WAVES_GE;
+constexpr int kPlainWaves1D = GS_PLAIN_WAVES_1D, kPlainWavesSp = GS_PLAIN_WAVES_SP, kPlainWavesGe = GS_PLAIN_WAVES_GE;
 
 static const void *traj_kernel_ptr(int model, bool batch) {
     switch (model) {

@@ -104,11 +104,13 @@ struct DevOps {
     __device__ static double fma3(double a, double b, double c) { return fma(a, b, c); }
     __device__ static double fmac(double a, double b, double c) { return fma(a, b, c); }
     __device__ static double inv(double v) { return rcp(v); }
+    __device__ static double root(double v) { return sqrt_nn(v); }
 };
 struct HostOps {
     static double fma3(double a, double b, double c) { return std::fma(a, b, c); }
     static double fmac(double a, double b, double c) { return std::fma(a, b, c); }
     static double inv(double v) { return 1.0 / v; }
+    static double root(double v) { return std::sqrt(v); }
 };
 
 // ln((1+s)/(1-s)) = 2s + 2s w R(w), w = s^2 <= 0.0295.
@@ -206,21 +208,9 @@ __host__ __device__ inline void sincos_small(double x, double &s, double &c) {
     c = ((q + 1u) & 2u) ? -cc : cc;
 }
 
-// atan x.  |x| > 1 uses pi/2 - atan(1/|x|); then a in [0, 1] is moved to
-// t = (a - c) / (1 + c a) with c in {0, tan(pi/8), 1}, so that
-// |t| <= tan(pi/16), and atan a = atan c + atan t with
-// atan t = t - t w A(w), w = t^2 (A: gen_poly.py, 3.2e-19).
+// atan t = t - t w A(w), w = t^2, |t| <= tan(pi/16) (A: gen_poly.py, 3.2e-19).
 template <class Ops>
-__host__ __device__ inline double atan_e(double x) {
-    const double ax = fabs(x);
-    const bool big = ax > 1.0;
-    const double a = big ? (ax < INFINITY ? Ops::inv(ax) : 0.0) : ax;   // device rcp(inf) is NaN
-    const bool low = a < 0.19891236737965801;       // tan(pi/16)
-    const bool high = a >= 0.66817863791929892;     // tan(3 pi/16)
-    const double cp = high ? 1.0 : (low ? 0.0 : 0.41421356237309503);          // tan(pi/8)
-    const double base_hi = high ? 0.78539816339744828 : (low ? 0.0 : 0.39269908169872414);
-    const double base_lo = high ? 3.061616997868383e-17 : (low ? 0.0 : 1.5308084989341915e-17);
-    const double t = low ? a : (a - cp) * Ops::inv(Ops::fma3(cp, a, 1.0));
+__host__ __device__ inline double atan_reduced(double t) {
     const double w = t * t;
     double p = -0.05115745967981170892;
     p = Ops::fmac(p, w, 0.06618732486766733679356);
@@ -230,10 +220,77 @@ __host__ __device__ inline double atan_e(double x) {
     p = Ops::fmac(p, w, 0.1428571428427355230372);
     p = Ops::fmac(p, w, -0.1999999999999728936999);
     p = Ops::fmac(p, w, 0.3333333333333333249568);
-    double r = Ops::fma3(-(t * w), p, t);
+    return Ops::fma3(-(t * w), p, t);
+}
+
+constexpr double kTanPi16 = 0.19891236737965801;
+constexpr double kTan3Pi16 = 0.66817863791929892;
+constexpr double kTanPi8 = 0.41421356237309503;
+constexpr double kPi8Hi = 0.39269908169872414, kPi8Lo = 1.5308084989341915e-17;   // pi/8
+constexpr double kPi4Hi = 0.78539816339744828, kPi4Lo = 3.061616997868383e-17;    // pi/4
+constexpr double kPi2Hi = 1.5707963267948966, kPi2Lo = 6.123233995736766e-17;     // pi/2
+
+// atan x.  |x| > 1 uses pi/2 - atan(1/|x|); then a in [0, 1] is moved to
+// t = (a - c) / (1 + c a) with c in {0, tan(pi/8), 1}, so that
+// |t| <= tan(pi/16), and atan a = atan c + atan t.
+template <class Ops>
+__host__ __device__ inline double atan_e(double x) {
+    const double ax = fabs(x);
+    const bool big = ax > 1.0;
+    const double a = big ? (ax < INFINITY ? Ops::inv(ax) : 0.0) : ax;   // device rcp(inf) is NaN
+    const bool low = a < kTanPi16;
+    const bool high = a >= kTan3Pi16;
+    const double cp = high ? 1.0 : (low ? 0.0 : kTanPi8);
+    const double base_hi = high ? kPi4Hi : (low ? 0.0 : kPi8Hi);
+    const double base_lo = high ? kPi4Lo : (low ? 0.0 : kPi8Lo);
+    const double t = low ? a : (a - cp) * Ops::inv(Ops::fma3(cp, a, 1.0));
+    double r = atan_reduced<Ops>(t);
     r = base_hi + (r + base_lo);
-    if (big) r = 1.5707963267948966 - (r - 6.123233995736766e-17);
+    if (big) r = kPi2Hi - (r - kPi2Lo);
     return x < 0.0 ? -r : r;
+}
+
+// atan(n / d) for 0 <= n <= d, d > 0, with a single reciprocal: the same
+// reduction in fraction form, t = (n - c d) / (d + c n).  The interval tests
+// compare n with c d (no quotient); near a boundary either side's t stays
+// within the polynomial's range.
+template <class Ops>
+__host__ __device__ inline double atan_frac(double n, double d) {
+    const bool low = n < kTanPi16 * d;
+    const bool high = n >= kTan3Pi16 * d;
+    const double cp = high ? 1.0 : (low ? 0.0 : kTanPi8);
+    const double base_hi = high ? kPi4Hi : (low ? 0.0 : kPi8Hi);
+    const double base_lo = high ? kPi4Lo : (low ? 0.0 : kPi8Lo);
+    const double t = Ops::fma3(-cp, d, n) * Ops::inv(Ops::fma3(cp, n, d));
+    return base_hi + (atan_reduced<Ops>(t) + base_lo);
+}
+
+// Neutral-sheet drift factors of the 2D models (SolarPropLikeCpuModel.cpp:
+// 183-209, GeliosphereCpuModel.cpp:335-361).  With x = sy / (pi - sy),
+// alphaH = acos x, arg = k tan(alphaH), k = 1 - 2 theta / pi:
+//   f = atan(arg) / alphaH,  g = tan(alphaH) / (alphaH (1 + arg^2)).
+// In X = sy and Y = sqrt(pi (pi - 2 sy)) = sqrt(1 - x^2) (pi - sy), with
+// Y >= 1.86 > |X| for |sy| <= 1: alphaH = pi/2 - atan(X / Y),
+// tan(alphaH) = Y / X, atan(arg) = atan(k Y / X) and
+// tan(alphaH) / (1 + arg^2) = X Y / (X^2 + k^2 Y^2).  None of the reference's
+// quotients (x, tan, arg, 1/alphaH, 1/(1 + arg^2)) is formed on its own: two
+// atan_frac reciprocals and one of alphaH (X^2 + k^2 Y^2) for both results.
+// At sy = 0 this gives the limits f = +-(pi/2)/alphaH, g = 0.
+template <class Ops>
+__host__ __device__ inline void sheet_drift(double sy, double k, double &f, double &g) {
+    constexpr double kPiD = 3.141592653589793;
+    const double X = sy, ax = fabs(sy);
+    const double Y = Ops::root(kPiD * (kPiD - 2.0 * sy));
+    const double a0 = atan_frac<Ops>(ax, Y);
+    const double aH = kPi2Hi - ((X < 0.0 ? -a0 : a0) - kPi2Lo);
+    const double kY = fabs(k) * Y;
+    const bool swap = kY > ax;
+    const double a1 = atan_frac<Ops>(swap ? ax : kY, swap ? kY : ax);
+    const double at = swap ? kPi2Hi - (a1 - kPi2Lo) : a1;     // atan(|k| Y / |X|)
+    const double D2 = Ops::fma3(X, X, kY * kY);
+    const double R = Ops::inv(aH * D2);
+    f = ((k < 0.0) != (X < 0.0) ? -at : at) * (D2 * R);      // atan(arg) / alphaH
+    g = (X * Y) * R;                                          // (X Y / D2) / alphaH
 }
 
 // ------------------------------------------------------------ Box-Muller
@@ -299,6 +356,21 @@ __device__ __forceinline__ double sind(double x) {
     return s;
 }
 __device__ __forceinline__ double atand(double x) { return atan_e<DevOps>(x); }
+__device__ __forceinline__ double atan_fracd(double n, double d) { return atan_frac<DevOps>(n, d); }
+
+// 1/a, 1/b, 1/c, 1/d from one reciprocal of abcd (a few ulp each; the
+// product must stay a normal number).
+__device__ __forceinline__ void rcp4(double a, double b, double c, double d, double &ia, double &ib, double &ic,
+                                     double &id) {
+    const double ab = a * b, abc = ab * c;
+    const double R = rcp(abc * d);
+    id = abc * R;
+    const double t = d * R;    // 1/(abc)
+    ic = ab * t;
+    const double u = c * t;    // 1/(ab)
+    ib = a * u;
+    ia = b * u;
+}
 
 }  // namespace fm
 }  // namespace gs

@@ -21,7 +21,9 @@ EXPORTS = ("gs_abi_version", "gs_create", "gs_destroy", "gs_last_error", "gs_def
 ORDER_ID, ORDER_ENERGY = 0, 1
 
 DBG_RCP, DBG_RSQRT, DBG_SQRT, DBG_LOG, DBG_EXP, DBG_SIN_TURN, DBG_COS_TURN, DBG_LOG_U32, \
-    DBG_RAW_RCP, DBG_RAW_RSQ, DBG_RAW_SQRT, DBG_SIN, DBG_COS, DBG_ATAN = range(14)
+    DBG_RAW_RCP, DBG_RAW_RSQ, DBG_RAW_SQRT, DBG_SIN, DBG_COS, DBG_ATAN, DBG_ATAN_FRAC, \
+    DBG_SHEET_F, DBG_SHEET_G = range(17)
+TWO_ARG = (DBG_ATAN_FRAC, DBG_SHEET_F, DBG_SHEET_G)
 
 
 class GsError(RuntimeError):
@@ -78,9 +80,11 @@ def lib():
 
 
 def host_math(fn, xs):
-    """Host evaluation of the math layer's polynomial parts (no device)."""
-    n = len(xs)
-    x = (ctypes.c_double * n)(*xs)
+    """Host evaluation of the math layer's polynomial parts (no device);
+    two-argument functions as in Engine.debug_math."""
+    m = len(xs)
+    n = m // 2 if fn in TWO_ARG else m
+    x = (ctypes.c_double * m)(*xs)
     y = (ctypes.c_double * n)()
     rc = lib().gs_host_math(fn, x, y, n)
     if rc:
@@ -187,8 +191,11 @@ class Engine:
         return b.value, r.value
 
     def debug_math(self, fn, xs):
-        n = len(xs)
-        x = (ctypes.c_double * n)(*xs)
+        """fn over xs on the device; the two-argument functions (TWO_ARG) take
+        xs = first arguments + second arguments (2n values), n results."""
+        m = len(xs)
+        n = m // 2 if fn in TWO_ARG else m
+        x = (ctypes.c_double * m)(*xs)
         y = (ctypes.c_double * n)()
         self._check(lib().gs_debug_math(self._ctx, fn, x, y, n), "gs_debug_math")
         return list(y)

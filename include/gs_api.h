@@ -253,6 +253,11 @@ typedef enum gs_debug_fn {
     GS_DBG_SIN = 11,      /* sin x, |x| < 2^20                         */
     GS_DBG_COS = 12,      /* cos x, |x| < 2^20                         */
     GS_DBG_ATAN = 13,     /* atan x                                    */
+    GS_DBG_ATAN_FRAC = 14, /* atan(x[i] / x[n + i]), 0 <= x[i] <= x[n + i]:
+                              x holds 2n values                          */
+    GS_DBG_SHEET_F = 15,  /* 2D neutral-sheet drift factor f(sy = x[i],
+                             k = x[n + i]) = atan(arg) / alphaH; 2n values */
+    GS_DBG_SHEET_G = 16,  /* the same pair's tan(alphaH) / (alphaH (1 + arg^2)) */
     GS_DBG_NORMALS = 100  /* internal: normal stream (gs_debug_normals) */
 } gs_debug_fn;
 

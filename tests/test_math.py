@@ -41,6 +41,56 @@ def samples(seed=1):
     return log_x, exp_x, u32, trig, at
 
 
+def frac_samples(seed):
+    """(n, d) with 0 <= n <= d: random ratios, the reduction's interval edges
+    (tan(pi/16), tan(3 pi/16)) from both sides, 0 and 1, over wide scales."""
+    rng = random.Random(seed)
+    pairs = []
+    for _ in range(600):
+        d = 10.0 ** rng.uniform(-10, 10)
+        pairs.append((d * rng.random(), d))
+    for q in (0.19891236737965801, 0.66817863791929892):
+        for e in (-3e-16, 0.0, 3e-16, 1e-9, -1e-9):
+            pairs.append((q * (1 + e), 1.0))
+    pairs += [(0.0, 1.0), (1.0, 1.0), (3.0, 3.0), (1e-300, 1.0), (0.5, 1.0)]
+    return pairs
+
+
+def check_atan_frac(fn_eval, pairs):
+    y = fn_eval(E.DBG_ATAN_FRAC, [p[0] for p in pairs] + [p[1] for p in pairs])
+    for a, (num, den) in zip(y, pairs):
+        ref = mpmath.atan(mpmath.mpf(num) / mpmath.mpf(den))
+        assert ulps(a, ref) <= 2.0, (num, den, a, float(ref))
+
+
+def sheet_samples(seed):
+    """(sin of the sheet angle, k = 1 - 2 theta / pi) pairs: random, both
+    signs, tiny and edge values."""
+    rng = random.Random(seed)
+    pairs = [(rng.uniform(-1, 1), rng.uniform(-1, 1)) for _ in range(400)]
+    pairs += [(s * 10.0 ** rng.uniform(-12, 0), rng.uniform(-1, 1)) for s in (1, -1) for _ in range(50)]
+    pairs += [(1.0, 1.0), (-1.0, -1.0), (1.0, -1.0), (0.5, 1e-9), (-0.5, -1e-9), (1e-9, 1.0), (0.3, 0.0)]
+    return pairs
+
+
+def check_sheet(fn_eval, pairs):
+    """The drift factors in fraction form against the reference's own chain
+    (SolarPropLikeCpuModel.cpp:183-209) evaluated at 50 digits."""
+    xs = [p[0] for p in pairs] + [p[1] for p in pairs]
+    f = fn_eval(E.DBG_SHEET_F, xs)
+    g = fn_eval(E.DBG_SHEET_G, xs)
+    for a, b, (sy, k) in zip(f, g, pairs):
+        sy, k = mpmath.mpf(sy), mpmath.mpf(k)
+        x = sy / (mpmath.pi - sy)
+        ah = mpmath.acos(x)
+        tanh = mpmath.tan(ah)
+        arg = k * tanh
+        rf = mpmath.atan(arg) / ah
+        rg = tanh / (ah * (1 + arg * arg))
+        assert abs(a - float(rf)) <= 8 * math.ulp(float(rf)) + 1e-300, (sy, k, a, float(rf))
+        assert abs(b - float(rg)) <= 8 * math.ulp(float(rg)) + 1e-300, (sy, k, b, float(rg))
+
+
 def check_trig(fn_eval, trig, at):
     for fn, f in ((E.DBG_SIN, mpmath.sin), (E.DBG_COS, mpmath.cos)):
         y = fn_eval(fn, trig)
@@ -69,11 +119,15 @@ def check(fn_eval, log_x, exp_x, u32, trig, at):
 
 def test_host_math_accuracy():
     check(E.host_math, *samples(1))
+    check_atan_frac(E.host_math, frac_samples(1))
+    check_sheet(E.host_math, sheet_samples(1))
 
 
 @pytest.mark.gpu
 def test_device_math_accuracy(engine):
     check(engine.debug_math, *samples(2))
+    check_atan_frac(engine.debug_math, frac_samples(2))
+    check_sheet(engine.debug_math, sheet_samples(2))
     rng = random.Random(3)
     xs = [10.0 ** rng.uniform(-200, 200) for _ in range(1000)] + [0.3, 1.0, 99.9999]
     for fn, f in ((E.DBG_RCP, lambda v: 1 / mpmath.mpf(v)), (E.DBG_RSQRT, lambda v: 1 / mpmath.sqrt(v)),
