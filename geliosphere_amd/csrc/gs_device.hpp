@@ -375,7 +375,8 @@ struct Sp {
     __device__ __forceinline__ bool step(const StepParams &Pm, const double *z, const Emit &emit) {
         double s, c;
         fm::sincosd(theta, s, c);
-        const double Rig = fm::sqrt_nn(T * (T + 2.0 * kT0));
+        const double Tp = T + 2.0 * kT0;
+        const double Rig = fm::sqrt_nn(T * Tp);
         const double r2 = r * r;
         const double gamma = r * Pm.omega_v * s;
         const double g2 = gamma * gamma;
@@ -386,33 +387,36 @@ struct Sp {
         const double sq = fm::sqrt_nn(opg);
         const double inv_sq = sq * inv_opg;
         const double Rc = (Rig < 0.1) ? 0.1 : Rig;
-        const double Kpar = Pm.K0 * beta * Rc * (5.0 / 3.4) * r2 * inv_sq * (1.0 / 3.0);
+        const double kb = Pm.K0 * beta;
+        // K_par = K0 beta max(P, 0.1) (5/3.4) r^2 / (3 sqrt(1 + gamma^2))
+        const double Kpar = kb * Rc * r2 * inv_sq * ((5.0 / 3.4) / 3.0);
         const double Kper = Pm.ratio * Kpar;
-        const double Krr = Kper + (Kpar - Kper) * inv_opg;
-        const double Ktt = Kper;
-        const double kbr = Pm.K0 * beta * Rig;
-        const double dKrrTmp = 2.0 * r * Pm.omega2_v2 * s * s;
-        const double t1 = Pm.ratio * kbr * ((2.0 * r * sq) - (0.5 * r2 * dKrrTmp * inv_sq)) * inv_opg;
-        const double t2 = (1.0 - Pm.ratio) * kbr * ((2.0 * r * opg * sq) - (1.5 * r2 * dKrrTmp * sq)) *
-                          (inv_opg * inv_opg * inv_opg);
+        const double Krr = fma(Kpar - Kper, inv_opg, Kper);
+        // dK_rr/dr with r^2 dKrrTmp = 2 r gamma^2 (dKrrTmp = 2 r Omega^2 sin^2 / V^2):
+        //   t1 = ratio K0 beta P r (2 sq - gamma^2 / sq) / (1 + gamma^2)
+        //   t2 = (1 - ratio) K0 beta P r sq (2 (1 + gamma^2) - 3 gamma^2) / (1 + gamma^2)^3
+        const double kr = kb * Rig * r;
+        const double t1 = Pm.ratio * kr * fma(-g2, inv_sq, 2.0 * sq) * inv_opg;
+        const double t2 = (1.0 - Pm.ratio) * kr * sq * fma(-3.0, g2, 2.0 * opg) * (inv_opg * inv_opg * inv_opg);
         const double dKrr = (t1 + t2) * (5.0 / (3.0 * 3.4));
-        const double z1 = z[0];
-        double dr = ((-Pm.V) + (2.0 * Krr * inv_r) + dKrr) * Pm.dt + z1 * fm::sqrt_nn(Pm.two_dt * Krr);
-        const double z2 = z[1];
-        double dth = Ktt * c * inv_s * inv_r * inv_r * Pm.dt * inv_opg + z2 * fm::sqrt_nn(Pm.two_dt * Ktt) * inv_r;
-        const double dTkin = -Pm.cp * ((T + 2.0 * kT0) * inv_E) * T * inv_r;
-        // drifts
-        const double Larmor = (0.0225 / kA) * Rig * r2 * inv_sq;
-        const double sy = fm::sind(Pm.tilt + Larmor * (kPi / 90.0) * inv_r);
+        const double sKrr = fm::sqrt_nn(Pm.two_dt * Krr);
+        const double sKtt = fm::sqrt_nn(Pm.two_dt * Kper);
+        double dr = fma(fma(2.0 * Krr, inv_r, dKrr) - Pm.V, Pm.dt, z[0] * sKrr);
+        const double csr = c * inv_s * inv_r;
+        double dth = fma(Kper * csr * inv_r * inv_opg, Pm.dt, z[1] * sKtt * inv_r);
+        const double dTkin = -Pm.cp * (Tp * inv_E) * T * inv_r;
+        // drifts: sheet angle alphaM + 2 Larmor pi / (180 r), Larmor = (0.0225 / A) P r^2 / sq
+        const double sy = fm::sind(fma(Rig * r * inv_sq, (0.0225 / kA) * (kPi / 90.0), Pm.tilt));
         double f, g;
         fm::sheet_drift<fm::DevOps>(sy, 1.0 - theta * (2.0 / kPi), f, g);
-        const double dc = kKonvF * (2.0 / (3.0 * kA)) * Rig * beta * r * gamma * f * inv_opg * inv_opg;
+        const double rbg = Rig * beta * r * gamma * inv_opg;
+        const double dc = (kKonvF * (2.0 / (3.0 * kA))) * rbg * f * inv_opg;
         const double DriftR = Pm.polarity * dc * c * inv_s;
         const double DriftTheta = -Pm.pol_theta * dc * (2.0 + g2);
-        const double fprime = -(2.0 / kPi) * g;
-        const double DriftSheetR = Pm.polarity * kKonvF * (1.0 / (3.0 * kA)) * Rig * beta * r * gamma * fprime * inv_opg;
-        dr = dr + (DriftR + DriftSheetR) * Pm.dt;
-        dth = dth + DriftTheta * Pm.dt * inv_r;
+        // fprime = -(2/pi) g
+        const double DriftSheetR = Pm.polarity * (kKonvF * (1.0 / (3.0 * kA)) * (-2.0 / kPi)) * rbg * g;
+        dr = fma(DriftR + DriftSheetR, Pm.dt, dr);
+        dth = fma(DriftTheta * Pm.dt, inv_r, dth);
         r = r + dr;
         theta = theta + dth;
         if (theta < 0.0) theta = fabs(theta);
@@ -448,7 +452,8 @@ struct Ge {
         constexpr double kf = 1.0 / (3.0 * (5.0 / 3.4));   // as written in GeliosphereCpuModel.cpp:210-299
         double s, c;
         fm::sincosd(theta, s, c);
-        const double Rig = fm::sqrt_nn(T * (T + 2.0 * kT0));
+        const double Tp = T + 2.0 * kT0;
+        const double Rig = fm::sqrt_nn(T * Tp);
         const double r2 = r * r;
         double inv_E, inv_r, inv_s, inv_E_r;
         {
@@ -468,30 +473,34 @@ struct Ge {
         const double deltarh2 = deltarh * deltarh;
         const double gamma = r * Pm.omega_v * s;
         const double g2 = gamma * gamma;
-        const double Cb = 1.0 + g2 + r2 * deltarh2;
+        const double h = fma(r2, deltarh2, g2);       // Cb - 1
+        const double Cb = 1.0 + h;
         const double inv_Cb = fm::rcp(Cb);
         const double sqCb = fm::sqrt_nn(Cb);
         const double inv_sqCb = sqCb * inv_Cb;
         const double inv_Cb15 = inv_Cb * inv_sqCb;
         const double inv_Cb25 = inv_Cb * inv_Cb15;
         const double Rc = (Rig < 0.1) ? 0.1 : Rig;
-        const double Kpar = Pm.K0 * beta * Rc * (5.0 / 3.4) * r2 * inv_sqCb * (1.0 / 3.0);
+        const double kb = Pm.K0 * beta;
+        const double Kpar = kb * Rc * r2 * inv_sqCb * ((5.0 / 3.4) / 3.0);
         const double Kper = Pm.ratio * Kpar;
-        const double dK = Kpar - Kper;
-        const double Krr = Kper + dK * inv_Cb;
-        const double Ktt = Kper + r2 * deltarh2 * dK * inv_Cb;
-        const double Krt = deltarh * dK * r * inv_Cb;
-        const double inv_Ktt = fm::rcp(Ktt);
-        const double sq2Ktt = fm::sqrt_nn(2.0 * Ktt);
-        const double B11 = fm::sqrt_pos(2.0 * (Krr * Ktt - Krt * Krt) * inv_Ktt);
-        const double B12 = Krt * inv_Ktt * sq2Ktt;
+        const double dKc = (Kpar - Kper) * inv_Cb;    // (K_par - K_per) / Cb
+        const double Krr = Kper + dKc;
+        const double Ktt = fma(r2 * deltarh2, dKc, Kper);
+        const double Krt = deltarh * r * dKc;
+        // 1/Ktt and sqrt(2 Ktt) from one rsqrt
+        const double rK = fm::rsqrt(Ktt);
+        const double inv_Ktt = rK * rK;
+        const double sq2Ktt = (1.4142135623730951 * Ktt) * rK;
+        const double B11 = fm::sqrt_pos(2.0 * fma(Krr, Ktt, -(Krt * Krt)) * inv_Ktt);
+        const double B12 = (1.4142135623730951 * Krt) * rK;   // Krt sqrt(2 Ktt) / Ktt
         const double B22 = sq2Ktt * inv_r;
-        const double kbr = Pm.K0 * beta * Rig;
-        const double COmega = 2.0 * r * (Pm.omega2_v2 * s * s + deltarh2);
-        const double dKper = Pm.ratio * kbr * ((2.0 * r * sqCb) - (0.5 * r2 * COmega * inv_sqCb)) * kf * inv_Cb;
-        const double dKrr = dKper + (1.0 - Pm.ratio) * kbr *
-                                        ((2.0 * r * Cb * sqCb) - (1.5 * r2 * COmega * sqCb)) * kf *
-                                        (inv_Cb * inv_Cb * inv_Cb);
+        const double kbr = kb * Rig;                          // K0 beta P
+        const double kr = kbr * r;
+        // r^2 COmega = 2 r (gamma^2 + r^2 deltarh^2) = 2 r h
+        const double dKper = Pm.ratio * kr * fma(-h, inv_sqCb, 2.0 * sqCb) * kf * inv_Cb;
+        const double dKrr = fma((1.0 - Pm.ratio) * kr * sqCb, fma(-3.0, h, 2.0 * Cb) * kf * (inv_Cb * inv_Cb * inv_Cb),
+                                dKper);
         const double r2w = Pm.omega2_v2 * r2;
         double dKtt;
         if (band) {
@@ -510,31 +519,31 @@ struct Ge {
             const double dKtt4 = 3.0 * CKtt * inv_s2 * kf * inv_Cb25;
             dKtt = dKtt1 + dKtt2 * (dKtt3 - dKtt4);
         }
-        const double dKrtr = (1.0 - Pm.ratio) * kbr * deltarh * r2 * kf * inv_Cb25;
-        const double krtt0 = (1.0 - Pm.ratio) * kbr * r2 * r * (kf / kRh) * inv_Cb25;
+        const double q = (1.0 - Pm.ratio) * kbr * r2 * kf * inv_Cb25;
+        const double dKrtr = q * deltarh;
+        const double krtt0 = q * r * (1.0 / kRh);
         const double dKrtt = band ? -krtt0 * delta * 3.0 * g2 * c * inv_s
                                   : -krtt0 * Pm.delta0 * c * inv_s * inv_s *
                                         (1.0 - (2.0 * r2 * deltarh2) + (4.0 * g2));
-        const double z1 = z[0];
-        const double z2 = z[1];
-        double dr = ((-Pm.V) + (2.0 * Krr * inv_r) + dKrr) * Pm.dt + dKrtt * Pm.dt * inv_r +
-                    Krt * c * Pm.dt * inv_r * inv_s + (z1 * B11 + z2 * B12) * Pm.sqdt;
-        const double z3 = z[2];
-        double dth = Ktt * c * inv_s * inv_r * inv_r * Pm.dt + dKtt * Pm.dt * inv_r * inv_r +
-                     dKrtr * Pm.dt + 2.0 * Krt * Pm.dt * inv_r + z3 * B22 * Pm.sqdt;
-        const double dTkin = -Pm.cp * ((T + 2.0 * kT0) * inv_E) * T * inv_r;
-        // drifts (Kappl 2016 Eq. 26, 32-37 with Cb in place of 1 + gamma^2)
-        const double Larmor = (0.0225 / kA) * Rig * r2 * inv_sqCb;
-        const double sy = fm::sind(Pm.tilt + Larmor * (kPi / 90.0) * inv_r);
+        const double csi = c * inv_s;
+        double dr = fma(fma(fma(Krt, csi, 2.0 * Krr + dKrtt), inv_r, dKrr) - Pm.V, Pm.dt,
+                        fma(z[0], B11, z[1] * B12) * Pm.sqdt);
+        double dth = fma(fma(fma(Ktt, csi, dKtt), inv_r, 2.0 * Krt) * inv_r + dKrtr, Pm.dt,
+                         z[2] * B22 * Pm.sqdt);
+        const double dTkin = -Pm.cp * (Tp * inv_E) * T * inv_r;
+        // drifts (Kappl 2016 Eq. 26, 32-37 with Cb in place of 1 + gamma^2); sheet
+        // angle alphaM + 2 Larmor pi / (180 r), Larmor = (0.0225 / A) P r^2 / sqrt(Cb)
+        const double sy = fm::sind(fma(Rig * r * inv_sqCb, (0.0225 / kA) * (kPi / 90.0), Pm.tilt));
         double f, g;
         fm::sheet_drift<fm::DevOps>(sy, 1.0 - theta * (2.0 / kPi), f, g);
-        const double dc = kKonvF * (2.0 / (3.0 * kA)) * Rig * beta * r * gamma * f * inv_Cb * inv_Cb;
-        const double DriftR = Pm.polarity * dc * c * inv_s;
+        const double rbg = Rig * beta * r * gamma * inv_Cb;
+        const double dc = (kKonvF * (2.0 / (3.0 * kA))) * rbg * f * inv_Cb;
+        const double DriftR = Pm.polarity * dc * csi;
         const double DriftTheta = -Pm.polarity * dc * (2.0 + g2);
-        const double fprime = -(2.0 / kPi) * g;
-        const double DriftSheetR = Pm.polarity * kKonvF * (1.0 / (3.0 * kA)) * Rig * beta * r * gamma * fprime * inv_Cb;
-        dr = dr + (DriftR + DriftSheetR) * Pm.dt;
-        dth = dth + DriftTheta * Pm.dt * inv_r;
+        // fprime = -(2/pi) g
+        const double DriftSheetR = Pm.polarity * (kKonvF * (1.0 / (3.0 * kA)) * (-2.0 / kPi)) * rbg * g;
+        dr = fma(DriftR + DriftSheetR, Pm.dt, dr);
+        dth = fma(DriftTheta * Pm.dt, inv_r, dth);
         r = r + dr;
         theta = theta + dth;
         T = T - dTkin;
