@@ -3,8 +3,6 @@ computed at 60 digits with mpmath.  Prints the coefficients and the worst
 relative error of each fit on its interval.
 
   log:  ln((1+s)/(1-s)) = 2s + 2s w R(w),  w = s^2 in [0, (3-2sqrt2)^2]
-  sin:  sin a = a - a w S(w),              w = a^2 in [0, (pi/4)^2]
-  cos:  cos a = 1 - w/2 + w^2 C(w)
   atan: atan t = t - t w A(w),            w = t^2 in [0, tan(pi/16)^2]
   log1p: ln(1+r) = r + r^2 L(r),           r in [-R_LOG, R_LOG]
 
@@ -21,7 +19,6 @@ import mpmath as mp
 
 mp.mp.dps = 60
 S_MAX = 3 - 2 * mp.sqrt(2)          # |s| for m in [1/sqrt2, sqrt2]
-A_MAX = mp.pi / 4
 
 
 def fit(f, a, b, deg):
@@ -34,20 +31,6 @@ def R(w):
         return mp.mpf(1) / 3
     s = mp.sqrt(w)
     return (mp.log((1 + s) / (1 - s)) - 2 * s) / (2 * s * w)
-
-
-def S(w):
-    if w == 0:
-        return mp.mpf(1) / 6
-    a = mp.sqrt(w)
-    return (a - mp.sin(a)) / (a * w)
-
-
-def C(w):
-    if w == 0:
-        return mp.mpf(1) / 24
-    a = mp.sqrt(w)
-    return (mp.cos(a) - 1 + w / 2) / (w * w)
 
 
 T_MAX = mp.tan(mp.pi / 16)
@@ -129,11 +112,6 @@ def worst(kind, coeffs, n=4000):
             w = s * s
             p = sum(c * w ** k for k, c in enumerate(coeffs))
             v, ref = 2 * s + 2 * s * w * p, mp.log((1 + s) / (1 - s))
-        elif kind == "sin":
-            a = A_MAX * i / n
-            w = a * a
-            p = sum(c * w ** k for k, c in enumerate(coeffs))
-            v, ref = a - a * w * p, mp.sin(a)
         elif kind == "log1p":
             r = R_LOG * (2 * i - n) / n
             if r == 0:
@@ -144,11 +122,6 @@ def worst(kind, coeffs, n=4000):
             w = t * t
             p = sum(c * w ** k for k, c in enumerate(coeffs))
             v, ref = t - t * w * p, mp.atan(t)
-        else:
-            a = A_MAX * i / n
-            w = a * a
-            p = sum(c * w ** k for k, c in enumerate(coeffs))
-            v, ref = 1 - w / 2 + w * w * p, mp.cos(a)
         e = max(e, abs(v - ref) / abs(ref))
     return e
 
@@ -157,8 +130,7 @@ if __name__ == "__main__":
     if "--tables" in sys.argv:
         import os
         write_tables(os.path.join(os.path.dirname(os.path.abspath(__file__)), "gs_bm_tables.h"))
-    for kind, f, lo, hi, deg in (("log", R, 0, S_MAX ** 2, 6), ("sin", S, 0, A_MAX ** 2, 5),
-                                 ("cos", C, 0, A_MAX ** 2, 5), ("atan", A, 0, T_MAX ** 2, 8),
+    for kind, f, lo, hi, deg in (("log", R, 0, S_MAX ** 2, 6), ("atan", A, 0, T_MAX ** 2, 7),
                                  ("log1p", L, -R_LOG, R_LOG, 4)):
         c, _ = fit(f, lo, hi, deg)
         print(kind, "degree", deg, "worst rel err %.3e" % float(worst(kind, c)))

@@ -838,8 +838,8 @@ gs_status gs_host_math(int32_t fn, const double *x, double *y, uint64_t n) {
         case GS_DBG_SIN_TURN: fm::sincos_turn<H>((uint32_t)x[i], kBmTrigHost, s, c); y[i] = s; break;
         case GS_DBG_COS_TURN: fm::sincos_turn<H>((uint32_t)x[i], kBmTrigHost, s, c); y[i] = c; break;
         case GS_DBG_LOG_U32: y[i] = fm::log_u32<H>((uint32_t)x[i], kBmLogHost); break;
-        case GS_DBG_SIN: fm::sincos_small<H>(x[i], s, c); y[i] = s; break;
-        case GS_DBG_COS: fm::sincos_small<H>(x[i], s, c); y[i] = c; break;
+        case GS_DBG_SIN: fm::sincos_tab<H>(x[i], kBmTrigHost, s, c); y[i] = s; break;
+        case GS_DBG_COS: fm::sincos_tab<H>(x[i], kBmTrigHost, s, c); y[i] = c; break;
         case GS_DBG_ATAN: y[i] = fm::atan_e<H>(x[i]); break;
         case GS_DBG_ATAN_FRAC: y[i] = fm::atan_frac<H>(x[i], x[n + i]); break;
         case GS_DBG_SHEET_F: fm::sheet_drift<H>(x[i], x[n + i], s, c); y[i] = s; break;

@@ -275,7 +275,8 @@ __device__ void fold_one(const DevParams &P, uint64_t *lds, const DevEvent &e) {
 // ------------------------------------------------------------------ models
 // A step returns whether the particle goes on; events are handed to
 // emit(t_inj, t_exit, r, theta, sumac) where they happen, so that nothing of
-// an event stays live in registers on the common path.
+// an event stays live in registers on the common path.  `trig` is the
+// workgroup's LDS copy of the sin/cos table (the 2D models' sin and cos).
 
 // 1D B-p.  State: r [AU], P = pc/q [GV], beta (of the last computed T, which
 // a reflection does not restore, as in OneDimensionBpCpuModel.cpp:108-140).
@@ -294,7 +295,7 @@ struct Bp {
         return r < 100.0002;
     }
     template <class Emit>
-    __device__ __forceinline__ bool step(const StepParams &Pm, const double *z, const Emit &emit) {
+    __device__ __forceinline__ bool step(const StepParams &Pm, const double *z, const double *trig, const Emit &emit) {
         const double K2dt = Pm.k2dt * (beta * P);                     // 2 K dt
         const double inv_r = fm::rcp(r);
         const double Pn = fma(-(Pm.cp * P), inv_r, P);               // p - 2 V p dt / (3 r)
@@ -333,7 +334,7 @@ struct Fp {
         return true;
     }
     template <class Emit>
-    __device__ __forceinline__ bool step(const StepParams &Pm, const double *z, const Emit &emit) {
+    __device__ __forceinline__ bool step(const StepParams &Pm, const double *z, const double *trig, const Emit &emit) {
         const double K2dt = Pm.k2dt * (beta * R);
         const double inv_r = fm::rcp(r);
         const double dr = fma(K2dt, inv_r, z[0] * fm::sqrt_nn(K2dt)) + Pm.vdt;
@@ -372,9 +373,9 @@ struct Sp {
         return r < 100.0;
     }
     template <class Emit>
-    __device__ __forceinline__ bool step(const StepParams &Pm, const double *z, const Emit &emit) {
+    __device__ __forceinline__ bool step(const StepParams &Pm, const double *z, const double *trig, const Emit &emit) {
         double s, c;
-        fm::sincosd(theta, s, c);
+        fm::sincosd(theta, trig, s, c);
         const double Tp = T + 2.0 * kT0;
         const double Rig = fm::sqrt_nn(T * Tp);
         const double r2 = r * r;
@@ -406,7 +407,7 @@ struct Sp {
         double dth = fma(Kper * csr * inv_r * inv_opg, Pm.dt, z[1] * sKtt * inv_r);
         const double dTkin = -Pm.cp * (Tp * inv_E) * T * inv_r;
         // drifts: sheet angle alphaM + 2 Larmor pi / (180 r), Larmor = (0.0225 / A) P r^2 / sq
-        const double sy = fm::sind(fma(Rig * r * inv_sq, (0.0225 / kA) * (kPi / 90.0), Pm.tilt));
+        const double sy = fm::sind(fma(Rig * r * inv_sq, (0.0225 / kA) * (kPi / 90.0), Pm.tilt), trig);
         double f, g;
         fm::sheet_drift<fm::DevOps>(sy, 1.0 - theta * (2.0 / kPi), f, g);
         const double rbg = Rig * beta * r * gamma * inv_opg;
@@ -448,10 +449,10 @@ struct Ge {
         return r < 100.0;
     }
     template <class Emit>
-    __device__ __forceinline__ bool step(const StepParams &Pm, const double *z, const Emit &emit) {
+    __device__ __forceinline__ bool step(const StepParams &Pm, const double *z, const double *trig, const Emit &emit) {
         constexpr double kf = 1.0 / (3.0 * (5.0 / 3.4));   // as written in GeliosphereCpuModel.cpp:210-299
         double s, c;
-        fm::sincosd(theta, s, c);
+        fm::sincosd(theta, trig, s, c);
         const double Tp = T + 2.0 * kT0;
         const double Rig = fm::sqrt_nn(T * Tp);
         const double r2 = r * r;
@@ -533,7 +534,7 @@ struct Ge {
         const double dTkin = -Pm.cp * (Tp * inv_E) * T * inv_r;
         // drifts (Kappl 2016 Eq. 26, 32-37 with Cb in place of 1 + gamma^2); sheet
         // angle alphaM + 2 Larmor pi / (180 r), Larmor = (0.0225 / A) P r^2 / sqrt(Cb)
-        const double sy = fm::sind(fma(Rig * r * inv_sqCb, (0.0225 / kA) * (kPi / 90.0), Pm.tilt));
+        const double sy = fm::sind(fma(Rig * r * inv_sqCb, (0.0225 / kA) * (kPi / 90.0), Pm.tilt), trig);
         double f, g;
         fm::sheet_drift<fm::DevOps>(sy, 1.0 - theta * (2.0 / kPi), f, g);
         const double rbg = Rig * beta * r * gamma * inv_Cb;

@@ -148,8 +148,8 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
         else stage_event(stage, O, pid, t_inj, t_exit, r, theta, sumac);
     };
     auto do_step = [&](const double *z) -> bool {
-        if constexpr (kBatch) return m.step(lp, z, emit);
-        else return m.step(P0, z, emit);
+        if constexpr (kBatch) return m.step(lp, z, bm.trig, emit);
+        else return m.step(P0, z, bm.trig, emit);
     };
 
     for (;;) {
@@ -425,8 +425,8 @@ __global__ void debug_kernel(int fn, const double *x, double *y, uint64_t n, uin
     case GS_DBG_RAW_RCP: y[i] = __builtin_amdgcn_rcp(x[i]); break;
     case GS_DBG_RAW_RSQ: y[i] = __builtin_amdgcn_rsq(x[i]); break;
     case GS_DBG_RAW_SQRT: y[i] = __builtin_amdgcn_sqrt(x[i]); break;
-    case GS_DBG_SIN: fm::sincosd(x[i], s, c); y[i] = s; break;
-    case GS_DBG_COS: fm::sincosd(x[i], s, c); y[i] = c; break;
+    case GS_DBG_SIN: fm::sincosd(x[i], bm.trig, s, c); y[i] = s; break;
+    case GS_DBG_COS: fm::sincosd(x[i], bm.trig, s, c); y[i] = c; break;
     case GS_DBG_ATAN: y[i] = fm::atand(x[i]); break;
     case GS_DBG_ATAN_FRAC: y[i] = fm::atan_fracd(x[i], x[n + i]); break;
     case GS_DBG_SHEET_F: fm::sheet_drift<fm::DevOps>(x[i], x[n + i], s, c); y[i] = s; break;

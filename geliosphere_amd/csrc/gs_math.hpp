@@ -7,20 +7,21 @@
 //                     Goldschmidt steps;
 //   log               exponent split, s = (m-1)/(m+1), 2s + 2s w R(w);
 //   exp               k ln2 + r reduction (two-part ln2), Taylor to r^13;
-//   sincos_small      sin/cos of |x| < 2^20 (Cody-Waite pi/2 reduction);
 //   atan              three-point argument reduction + odd polynomial;
 //   sincos_turn       sin/cos(2 pi x 2^-32) for a 32-bit integer x and
 //   log_u32           ln((x + 1) 2^-32): the Box-Muller angle and radius
-//                     argument, table-driven (P. T. P. Tang, "Table-driven
+//                     argument, and
+//   sincos_tab        sin/cos of |x| < 2^20 (Cody-Waite pi/128 reduction),
+//                     table-driven (P. T. P. Tang, "Table-driven
 //                     implementation of the logarithm function in IEEE
 //                     floating-point arithmetic", ACM TOMS 16(4), 1990: a
 //                     table entry next to the argument and a short
 //                     polynomial in the remainder; the same scheme for
 //                     sin/cos).  The tables are gs_bm_tables.h; the kernels
 //                     keep them in LDS.
-// R, S, C, A, L are Chebyshev fits made by gen_poly.py (worst relative error
-// of the function value 4.5e-18, 1.4e-17, 6.9e-19, 3.2e-19, 3.7e-17 before
-// rounding).
+// R, A, L are Chebyshev fits made by gen_poly.py (worst relative error of the
+// function value 4.5e-18, 3.2e-19, 3.7e-17 before rounding); the sin/cos
+// remainder series are Taylor's, exact to 1e-17 on |a| <= pi/256.
 //
 // Every multiply-add is an explicit fused multiply-add (Ops::fma3 / fmac: on
 // the device v_fma_f64, on the host std::fma).  Both are correctly rounded,
@@ -169,45 +170,6 @@ __host__ __device__ inline double exp_e(double x) {
     return bits_to_double(double_to_bits(p) + ((uint64_t)ki << 52));
 }
 
-// sin a = a - a w S(w), cos a = 1 - w/2 + w^2 C(w), w = a^2, |a| <= pi/4.
-template <class Ops>
-__host__ __device__ inline void sincos_poly(double a, double &s, double &c) {
-    const double w = a * a;
-    double ps = -1.591812929486660866682e-10;
-    ps = Ops::fmac(ps, w, 2.505113184500362441675e-8);
-    ps = Ops::fmac(ps, w, -0.000002755731610255244011882);
-    ps = Ops::fmac(ps, w, 0.0001984126983675857432297);
-    ps = Ops::fmac(ps, w, -0.008333333333330948485545);
-    ps = Ops::fmac(ps, w, 0.1666666666666666462354);
-    s = Ops::fma3(-(a * w), ps, a);
-    double pc = -1.138263242552171800541e-11;
-    pc = Ops::fmac(pc, w, 2.087614626840319789335e-9);
-    pc = Ops::fmac(pc, w, -2.755731727172979288199e-7);
-    pc = Ops::fmac(pc, w, 0.00002480158729876568792727);
-    pc = Ops::fmac(pc, w, -0.001388888888888739723669);
-    pc = Ops::fmac(pc, w, 0.04166666666666666538875);
-    c = Ops::fma3(w * w, pc, Ops::fma3(-0.5, w, 1.0));
-}
-
-// sin and cos of x for |x| < 2^20: k = rint(x 2/pi), a = x - k pi/2 with
-// pi/2 in two parts (the first has 31 fraction bits, so k * kPio2Hi is
-// exact), |a| <= pi/4, then the quadrant k mod 4.
-constexpr double kTwoOverPi = 0.6366197723675814;
-constexpr double kPio2Hi = 1.5707963267341256;
-constexpr double kPio2Lo = 6.077100506506192e-11;
-template <class Ops>
-__host__ __device__ inline void sincos_small(double x, double &s, double &c) {
-    const double k = rint(x * kTwoOverPi);
-    const double a = Ops::fma3(-k, kPio2Lo, Ops::fma3(-k, kPio2Hi, x));
-    double sa, ca;
-    sincos_poly<Ops>(a, sa, ca);
-    const uint32_t q = (uint32_t)(int32_t)k & 3u;
-    const double ss = (q & 1u) ? ca : sa;
-    const double cc = (q & 1u) ? sa : ca;
-    s = (q & 2u) ? -ss : ss;
-    c = ((q + 1u) & 2u) ? -cc : cc;
-}
-
 // atan t = t - t w A(w), w = t^2, |t| <= tan(pi/16) (A: gen_poly.py, 3.2e-19).
 template <class Ops>
 __host__ __device__ inline double atan_reduced(double t) {
@@ -298,22 +260,40 @@ constexpr int kBmTrigWords = 512;   // (sin, cos)(2 pi j/256), j = 0..255
 constexpr int kBmLogWords = 512;    // 128 x (invc, -ln invc hi, lo, 0)
 constexpr uint64_t kLogOff = 0x3fe5f00000000000ull;   // bits of 0.685546875
 
+// sin/cos(A + a) from a table entry sA = sin A, cA = cos A and |a| <= pi/256
+// (w = a^2): sin a = a + a w (-1/6 + w/120), cos a - 1 = w (-1/2 + w/24 -
+// w^2/720) (next terms below 1e-17), then sin(A + a) = sA + (sA (cos a - 1) +
+// cA sin a) and cos(A + a) = cA + (cA (cos a - 1) - sA sin a).
+template <class Ops>
+__host__ __device__ inline void sincos_entry(double a, double sA, double cA, double &s, double &c) {
+    const double w = a * a;
+    const double sa = Ops::fma3(a * w, Ops::fmac(1.0 / 120.0, w, -1.0 / 6.0), a);
+    const double cm = w * Ops::fmac(Ops::fmac(-1.0 / 720.0, w, 1.0 / 24.0), w, -0.5);
+    s = Ops::fma3(cA, sa, Ops::fma3(sA, cm, sA));
+    c = Ops::fma3(-sA, sa, Ops::fma3(cA, cm, cA));
+}
+
 // sin/cos(2 pi x 2^-32): j = round(x 2^-24) mod 256, d = x - j 2^24 in
-// [-2^23, 2^23) exactly, a = d 2 pi 2^-32 (|a| <= pi/256, w = a^2);
-// sin a = a + a w (-1/6 + w/120), cos a - 1 = w (-1/2 + w/24 - w^2/720)
-// (next terms below 1e-17), then sin(A + a) = sA + (sA (cos a - 1) + cA sin a)
-// and cos(A + a) = cA + (cA (cos a - 1) - sA sin a) with sA, cA from `trig`.
+// [-2^23, 2^23) exactly, a = d 2 pi 2^-32, entry j of `trig`.
 template <class Ops>
 __host__ __device__ inline void sincos_turn(uint32_t x, const double *trig, double &s, double &c) {
     const uint32_t j = (x + 0x800000u) >> 24;
     const int32_t d = (int32_t)(x - (j << 24));
-    const double a = (double)d * kTwoPiOver2p32;
-    const double w = a * a;
-    const double sa = Ops::fma3(a * w, Ops::fmac(1.0 / 120.0, w, -1.0 / 6.0), a);
-    const double cm = w * Ops::fmac(Ops::fmac(-1.0 / 720.0, w, 1.0 / 24.0), w, -0.5);
-    const double sA = trig[2 * j], cA = trig[2 * j + 1];
-    s = Ops::fma3(cA, sa, Ops::fma3(sA, cm, sA));
-    c = Ops::fma3(-sA, sa, Ops::fma3(cA, cm, cA));
+    sincos_entry<Ops>((double)d * kTwoPiOver2p32, trig[2 * j], trig[2 * j + 1], s, c);
+}
+
+// sin and cos of x for |x| < 2^20 from the same table: k = rint(x 128/pi),
+// a = x - k pi/128 with pi/128 in two parts (the first has 27 significant
+// bits, so k times it is exact for |k| < 2^26), |a| <= pi/256, entry k mod 256.
+constexpr double k128OverPi = 40.74366543152521;
+constexpr double kPi128Hi = 0x1.921fb54p-6;
+constexpr double kPi128Lo = 0x1.10b4611a62633p-36;
+template <class Ops>
+__host__ __device__ inline void sincos_tab(double x, const double *trig, double &s, double &c) {
+    const double k = rint(x * k128OverPi);
+    const double a = Ops::fma3(-k, kPi128Lo, Ops::fma3(-k, kPi128Hi, x));
+    const uint32_t j = (uint32_t)(int32_t)k & 255u;
+    sincos_entry<Ops>(a, trig[2 * j], trig[2 * j + 1], s, c);
 }
 
 // ln((x + 1) 2^-32) for a 32-bit integer x, in [-22.18, 0].  v = x + 1
@@ -349,10 +329,12 @@ __device__ __forceinline__ double log_u32d(uint32_t x, const double *logt) { ret
 __device__ __forceinline__ void sincos_turnd(uint32_t x, const double *trig, double &s, double &c) {
     sincos_turn<DevOps>(x, trig, s, c);
 }
-__device__ __forceinline__ void sincosd(double x, double &s, double &c) { sincos_small<DevOps>(x, s, c); }
-__device__ __forceinline__ double sind(double x) {
+__device__ __forceinline__ void sincosd(double x, const double *trig, double &s, double &c) {
+    sincos_tab<DevOps>(x, trig, s, c);
+}
+__device__ __forceinline__ double sind(double x, const double *trig) {
     double s, c;
-    sincos_small<DevOps>(x, s, c);
+    sincos_tab<DevOps>(x, trig, s, c);
     return s;
 }
 __device__ __forceinline__ double atand(double x) { return atan_e<DevOps>(x); }

@@ -34,7 +34,12 @@ def samples(seed=1):
           [float((j << 24) + d) for j in range(0, 256, 5) for d in (-(1 << 23) - 1, -(1 << 23), (1 << 23) - 1)
            if 0 <= (j << 24) + d < 2 ** 32] + \
           [float((1 << k) + d) for k in range(1, 32) for d in (-2, -1, 0)]
+    # sin/cos: the 2D steps' ranges (theta in [0, pi], sheet angles of tens of
+    # radians), larger arguments, and both sides of the table's interval
+    # edges (k + 1/2) pi/128
     trig = [rng.uniform(-10, 10) for _ in range(400)] + [rng.uniform(0, math.pi) for _ in range(400)] + \
+           [rng.uniform(-1e5, 1e5) for _ in range(100)] + \
+           [(k + 0.5) * math.pi / 128 * (1 + e) for k in range(0, 300, 7) for e in (-1e-15, 1e-15)] + \
            [0.0, math.pi / 4, math.pi / 2, math.pi, 2 * math.pi, 1e-10, -3.0, 1000.5]
     at = [s * 10.0 ** rng.uniform(-8, 8) for s in (1, -1) for _ in range(400)] + \
          [0.0, 1.0, -1.0, 0.19891236737965801, 0.66817863791929892, 0.4142135623730950, 1e300, -1e300]
