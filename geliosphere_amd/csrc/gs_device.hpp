@@ -305,14 +305,24 @@ struct Bp {
         const double E2 = fma(Pn, Pn, kT0 * kT0);
         const double invE = fm::rsqrt(E2);
         const double bn = Pn * invE;
-        const bool stop = bn < kBetaMinBp;
-        const bool out = bn > kBetaMinBp && E2 < kE2Exit && rn > 100.0 && rb < 100.0;
-        if (out) emit(t_inj, fma(E2, invE, -kT0), rn, 0.0, 0.0);
+        // Goes on while beta >= 0.01 (false for a NaN state, which ends it as
+        // r < 100.0002 does in the reference loop).
+        bool alive = bn >= kBetaMinBp;
         beta = bn;
-        const bool refl = rn < kReflectBp;   // a reflected step keeps P
+        if (rn > 100.0) {
+            // at the outer boundary (no reflection possible): exit or end
+            const bool out = bn > kBetaMinBp && E2 < kE2Exit && rb < 100.0;
+            if (out) emit(t_inj, fma(E2, invE, -kT0), rn, 0.0, 0.0);
+            r = rn;
+            P = Pn;
+            return alive && !out && rn < 100.0002;
+        }
+        // r stays below 100.0002: rn <= 100, and a reflection restores the
+        // previous r; a reflected step keeps P
+        const bool refl = rn < kReflectBp;
         r = refl ? rb : rn;
         P = refl ? P : Pn;
-        return !(out || stop) && r < 100.0002;
+        return alive;
     }
 };
 
@@ -343,10 +353,13 @@ struct Fp {
         double rn = r + dr;
         const double E2 = fma(Pn, Pn, kT0 * kT0);
         const double invE = fm::rsqrt(E2);
-        if (beta > 0.01 && E2 < kE2Cross) {
-            // (r - 1) / ((r - dr) - 1) < 0, without the division
-            const double a = rn - 1.0, b = (rn - dr) - 1.0;
-            if ((a < 0.0 && b >= 0.0) || (a > 0.0 && b < 0.0)) emit(t_inj, fma(E2, invE, -kT0), rn, 0.0, sumac);
+        const double rb = rn - dr;
+        if (fmin(rn, rb) < 1.0) {
+            // a crossing of 1 AU needs one of rn, r - dr below 1:
+            // (rn - 1) / ((rn - dr) - 1) < 0, without the division
+            const double a = rn - 1.0, b = rb - 1.0;
+            if (beta > 0.01 && E2 < kE2Cross && ((a < 0.0 && b >= 0.0) || (a > 0.0 && b < 0.0)))
+                emit(t_inj, fma(E2, invE, -kT0), rn, 0.0, sumac);
         }
         const bool stop = beta < 0.01;
         beta = Pn * invE;
