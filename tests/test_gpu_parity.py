@@ -66,18 +66,18 @@ def fractions_within(diffs):
     return [sum(d < t for d in diffs) / max(len(diffs), 1) for t in TOLS]
 
 
-@pytest.mark.parametrize("model", MODELS)
-def test_same_seed_particle_parity(engine, oracle, model):
-    """1D models: every event agrees to 1e-9 (their steps use only +,-,*,/,
-    sqrt).  2D models add sin, cos, acos, atan, whose GPU and libm results
-    differ in the last bit, and their trajectories amplify last-bit
-    differences (most in the Geliosphere polar branch).  There the GPU must
-    stay as close to the oracle as the oracle stays to itself when every
-    normal is perturbed by one ulp: the two distributions of per-particle
-    differences must agree within binomial noise at every tolerance."""
-    n = SAME_SEED_N[model]
-    p = abi.make_params(model, N=n, seed=20240607, weights=ALL_WEIGHTS, tilt_deg=45.0)
-    say("same-seed parity", abi.MODEL_NAMES[model], "N", n)
+def same_seed_check(engine, oracle, p, label):
+    """1D models: every event agrees to 1e-8 and step counts are identical
+    (their steps use only +,-,*,/,sqrt).  2D models add sin, cos, acos,
+    atan, whose GPU and libm results differ in the last bit, and their
+    trajectories amplify last-bit differences (most in the Geliosphere polar
+    branch).  There the GPU must stay as close to the oracle as the oracle
+    stays to itself when every normal is perturbed by one ulp: the two
+    distributions of per-particle differences must agree within binomial
+    noise at every tolerance."""
+    n = p.n_particles
+    model = p.model
+    say("same-seed parity", label, "N", n)
     t = time.time()
     gs, grecs, gn = engine.run_plain(p, records=True, rec_cap=4 * n + 64)
     tg = time.time() - t
@@ -93,10 +93,10 @@ def test_same_seed_particle_parity(engine, oracle, model):
         assert fg[2] == 1.0 and gn == on and gs.total_steps == os_.total_steps
         for b in range(p.n_bins):
             assert gs.count[b] == os_.count[b]
-            for k in range(len(ALL_WEIGHTS)):
+            for k in range(p.n_weights):
                 assert gs.sum_w[k][b] == pytest.approx(os_.sum_w[k][b], rel=1e-9, abs=0.0)
         return
-    q = abi.clone_params(p, flags=ORACLE_FLAG_PERTURB)
+    q = abi.clone_params(p, flags=p.flags | ORACLE_FLAG_PERTURB)
     rc, ps, precs, pn = oracle.run(q, n_threads=0, records=True, rec_cap=4 * n + 64)
     fp = fractions_within(record_diffs(precs, orecs))
     say("  oracle vs 1-ulp within " + " ".join("%.0e:%.4f" % (t, f) for t, f in zip(TOLS, fp)))
@@ -105,6 +105,38 @@ def test_same_seed_particle_parity(engine, oracle, model):
         assert a >= b - margin
     assert gn == on
     assert rel(gs.total_steps, os_.total_steps) < 0.01
+
+
+@pytest.mark.parametrize("model", MODELS)
+def test_same_seed_particle_parity(engine, oracle, model):
+    p = abi.make_params(model, N=SAME_SEED_N[model], seed=20240607, weights=ALL_WEIGHTS, tilt_deg=45.0)
+    same_seed_check(engine, oracle, p, abi.MODEL_NAMES[model])
+
+
+# Other injection modes, shard offsets, injection points, polarities and the
+# reference GPU's SolarProp drift variant, particle by particle.
+SAME_SEED_CASES = [
+    ("1D B-p grid injection, shard offset, r=5 AU, dt=100 s",
+     dict(model=abi.MODEL_1D_BP, N=3000, injection=abi.INJ_GRID, grid_n=997, first=1_000_003, r_inj=5.0, dt=100.0)),
+    ("1D F-p grid injection, V=600 km/s",
+     dict(model=abi.MODEL_1D_FP, N=1_000_000, injection=abi.INJ_GRID, grid_n=5000, V_kms=600.0)),
+    ("SolarProp bins, reference GPU drift, A<0, tilt 75, theta 60, r=3 AU",
+     dict(model=abi.MODEL_2D_SOLARPROP, N=1200, injection=abi.INJ_SOLARPROP_BINS,
+          flags=abi.FLAG_SOLARPROP_GPU_DRIFT, polarity=-1.0, tilt_deg=75.0, theta_deg=60.0, r_inj=3.0)),
+    ("Geliosphere bins, A<0, polar injection theta 1 deg",
+     dict(model=abi.MODEL_2D_GELIOSPHERE, N=1000, injection=abi.INJ_SOLARPROP_BINS, polarity=-1.0,
+          tilt_deg=20.0, theta_deg=1.0)),
+    ("Geliosphere log-uniform, r=30 AU, theta 150, shard offset",
+     dict(model=abi.MODEL_2D_GELIOSPHERE, N=1000, r_inj=30.0, theta_deg=150.0, tilt_deg=60.0, first=777_777)),
+]
+
+
+@pytest.mark.parametrize("label,kw", SAME_SEED_CASES, ids=[c[0] for c in SAME_SEED_CASES])
+def test_same_seed_particle_parity_configs(engine, oracle, label, kw):
+    kw = dict(kw)
+    model = kw.pop("model")
+    p = abi.make_params(model, seed=424242, weights=ALL_WEIGHTS, **kw)
+    same_seed_check(engine, oracle, p, label)
 
 
 STAT_CASES = [
