@@ -60,6 +60,10 @@ struct gs_ctx {
     uint32_t *d_order = nullptr;        // permutation, cap_order entries
     uint64_t cap_order = 0;
     uint32_t *d_order_counts = nullptr;  // 2 * kOrderMaxBuckets words
+    uint32_t *d_border = nullptr;        // batches: permutation over the virtual ids
+    uint64_t cap_border = 0;
+    uint32_t *d_bcounts = nullptr;       // batches: 2 * kOrderSimBuckets words per simulation
+    uint64_t cap_bcounts = 0;
     uint32_t block_override[4] = {0, 0, 0, 0};
     uint32_t bpcu_override[4] = {0, 0, 0, 0};
     gs_run_stats stats{};
@@ -185,6 +189,20 @@ gs_status ensure_order(gs_ctx *ctx, uint64_t n) {
     if (ctx->d_order) { (void)hipFree(ctx->d_order); ctx->d_order = nullptr; ctx->cap_order = 0; }
     HIP_TRY(ctx, hipMalloc((void **)&ctx->d_order, n * sizeof(uint32_t)));
     ctx->cap_order = n;
+    return GS_OK;
+}
+
+gs_status ensure_batch_order(gs_ctx *ctx, uint64_t n, uint64_t count_words) {
+    if (ctx->cap_border < n) {
+        if (ctx->d_border) { (void)hipFree(ctx->d_border); ctx->d_border = nullptr; ctx->cap_border = 0; }
+        HIP_TRY(ctx, hipMalloc((void **)&ctx->d_border, n * sizeof(uint32_t)));
+        ctx->cap_border = n;
+    }
+    if (ctx->cap_bcounts < count_words) {
+        if (ctx->d_bcounts) { (void)hipFree(ctx->d_bcounts); ctx->d_bcounts = nullptr; ctx->cap_bcounts = 0; }
+        HIP_TRY(ctx, hipMalloc((void **)&ctx->d_bcounts, count_words * sizeof(uint32_t)));
+        ctx->cap_bcounts = count_words;
+    }
     return GS_OK;
 }
 
@@ -391,6 +409,7 @@ gs_status run_sub_batch(gs_ctx *ctx, const gs_params *sims, const std::vector<ui
     for (size_t j = 0; j < m; ++j) {
         const gs_params &p = sims[order[j]];
         ctx->h_params[j] = derive(p);
+        if (ctx->order_mode == GS_ORDER_ENERGY) ctx->h_params[j].order = order_key(p, ctx->h_params[j]);
         room[j] = event_room(p.model, p.n_particles);
         total_room += room[j];
         max_nb = std::max(max_nb, p.n_bins);
@@ -407,15 +426,25 @@ gs_status run_sub_batch(gs_ctx *ctx, const gs_params *sims, const std::vector<ui
         ctx->h_meta[m + j] = room[j];
         eo += room[j];
     }
-    for (const Launch &L : launches) {
+    std::vector<uint64_t> perm_at(nl), nv(nl);   // each launch's slice of the permutation
+    uint64_t perm_total = 0;
+    for (size_t l = 0; l < nl; ++l) {
+        const Launch &L = launches[l];
         uint64_t v = 0;
         for (uint32_t t = 0; t < L.count; ++t) {
             ctx->h_meta[L.off_pos + t] = v;
             v += sims[order[L.base + t]].n_particles;
         }
         ctx->h_meta[L.off_pos + L.count] = v;
+        perm_at[l] = perm_total;
+        nv[l] = v;
+        perm_total += v;
     }
-    gs_status st = ensure_sims(ctx, m);
+    const bool ordered = ctx->order_mode == GS_ORDER_ENERGY;
+    gs_status st = ordered ? ensure_batch_order(ctx, std::max<uint64_t>(perm_total, 1),
+                                                2 * (uint64_t)gs::kOrderSimBuckets * m)
+                           : GS_OK;
+    if (st == GS_OK) st = ensure_sims(ctx, m);
     if (st == GS_OK) st = ensure_params(ctx, m);
     if (st == GS_OK) st = ensure_events(ctx, std::max<uint64_t>(total_room, 1));
     if (st == GS_OK) st = ensure_meta(ctx, std::max<size_t>(meta_words, 64));
@@ -427,7 +456,8 @@ gs_status run_sub_batch(gs_ctx *ctx, const gs_params *sims, const std::vector<ui
     const unsigned fb = std::max(1u, std::min(fold_blocks(max_room), (unsigned)std::max<size_t>(1, 4096 / m)));
     // ---- graph (re)build when the layout differs from the cached one
     std::vector<uint64_t> key = {m, nl, (uint64_t)(uintptr_t)ctx->d_words, (uint64_t)(uintptr_t)ctx->d_events,
-                                 (uint64_t)(uintptr_t)ctx->d_params, (uint64_t)(uintptr_t)ctx->d_meta, lds, fb};
+                                 (uint64_t)(uintptr_t)ctx->d_params, (uint64_t)(uintptr_t)ctx->d_meta, lds, fb,
+                                 ordered, (uint64_t)(uintptr_t)ctx->d_border, (uint64_t)(uintptr_t)ctx->d_bcounts};
     for (const Launch &L : launches) {
         key.push_back((uint64_t)L.model);
         key.push_back(L.count);
@@ -467,6 +497,13 @@ gs_status run_sub_batch(gs_ctx *ctx, const gs_params *sims, const std::vector<ui
             O.queue = ctx->d_meta + q_base + l;
             O.cycles = ctx->d_meta + c_base + l;
             O.order = nullptr;
+            if (ordered) {
+                e = gs_launch_batch_order(ctx->d_params + L.base, L.count, ctx->d_meta + L.off_pos, nv[l],
+                                          ctx->d_bcounts + 2 * (size_t)gs::kOrderSimBuckets * L.base,
+                                          ctx->d_border + perm_at[l], sg);
+                if (e != hipSuccess) break;
+                O.order = ctx->d_border + perm_at[l];
+            }
             O.g_count = W.count + (size_t)L.base * GS_MAX_BINS;
             O.g_digits = W.digits + (size_t)L.base * gs::kDigitsPerSim;
             O.g_stats = W.stats + (size_t)L.base * gs::ST_COUNT_;
@@ -579,6 +616,8 @@ void gs_destroy(gs_ctx *ctx) {
     if (ctx->d_meta) (void)hipFree(ctx->d_meta);
     if (ctx->d_order) (void)hipFree(ctx->d_order);
     if (ctx->d_order_counts) (void)hipFree(ctx->d_order_counts);
+    if (ctx->d_border) (void)hipFree(ctx->d_border);
+    if (ctx->d_bcounts) (void)hipFree(ctx->d_bcounts);
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     for (int i = 0; i < 3; ++i) {
         if (ctx->side[i]) (void)hipStreamDestroy(ctx->side[i]);

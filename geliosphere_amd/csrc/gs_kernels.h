@@ -46,8 +46,20 @@ struct StepParams {
     double pol_theta;     // polarity used in the 2D theta drift
 };
 
+// Scheduling order of a run's particles (gs_launch_order, batches): offsets
+// sorted by a bucket of their injection energy, bucket = (bits(T) >> shift)
+// - kmin clamped to [0, nb); the bits of a positive double grow with its
+// value.  nb = 0: id order.  nb <= kOrderSimBuckets.
+constexpr uint32_t kOrderMaxBuckets = 1024;
+constexpr uint32_t kOrderSimBuckets = 512;
+struct OrderKey {
+    uint32_t shift, nb;
+    uint64_t kmin;
+};
+
 // Parameters of one simulation.
 struct DevParams : StepParams {
+    OrderKey order;       // batches: this simulation's energy buckets
     int32_t model, injection;
     uint32_t flags, n_bins, n_weights, grid_n;
     int32_t weights[GS_MAX_WEIGHTS];
@@ -84,16 +96,8 @@ struct DevOut {
     const uint64_t *ev_cap;
     uint64_t *n_events;       // per simulation; may exceed ev_cap (then re-run with more room)
     uint64_t *cycles;         // wave cycles issued by the launch
-    const uint32_t *order;    // plain runs: queue position -> particle offset (nullptr: identity)
-};
-
-// Scheduling order of a plain run (gs_launch_order): particle offsets sorted
-// by a bucket of their injection energy, bucket = (bits(T) >> shift) - kmin
-// clamped to [0, nb); the bits of a positive double grow with its value.
-constexpr uint32_t kOrderMaxBuckets = 1024;
-struct OrderKey {
-    uint32_t shift, nb;
-    uint64_t kmin;
+    const uint32_t *order;    // queue position (plain) or virtual id (batch) -> particle offset
+                              // within its simulation (nullptr: identity)
 };
 
 // LDS layout per workgroup: count[n_bins] | digits[n_weights][2][n_bins][16] | stat[ST_COUNT_]
@@ -121,6 +125,13 @@ hipError_t gs_launch_traj(int model, bool batch, const gs::DevParams *sims, uint
 // scratch.
 hipError_t gs_launch_order(const gs::DevParams *sims, uint64_t n, const gs::OrderKey &key, uint32_t *counts,
                            uint32_t *perm, hipStream_t s);
+// Scheduling order of a batch launch: for the n_sims simulations of the
+// virtual id space `offsets` (device, n_sims + 1 prefix sums), perm[v]
+// receives a particle offset within v's simulation such that each
+// simulation's range is in ascending energy-bucket order (sims[s].order; id
+// order where nb = 0).  counts: 2 * n_sims * kOrderSimBuckets words.
+hipError_t gs_launch_batch_order(const gs::DevParams *sims, uint32_t n_sims, const uint64_t *offsets, uint64_t n,
+                                 uint32_t *counts, uint32_t *perm, hipStream_t s);
 // Histograms of n_sims simulations from their event regions; lds_bytes fits
 // the largest (n_bins, n_weights) among them.
 hipError_t gs_launch_fold(const gs::DevParams *sims, uint32_t n_sims, const gs::DevOut &O,

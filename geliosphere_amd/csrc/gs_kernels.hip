@@ -11,6 +11,8 @@
 // per thread and waits for the slowest particle of every launch
 // (CUDAKernel/src/OneDimensionBpModel.cu:171-184); here no lane waits for
 // another lane's particle except at the very end of the run.
+#include <algorithm>
+
 #include "gs_device.hpp"
 #include "gs_kernels.h"
 
@@ -213,7 +215,7 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
                         cap = (uint32_t)Q.max_steps;
                         first = Q.first_particle;
                     }
-                    pid = first + (take - sim_lo);
+                    pid = first + (O.order ? (uint64_t)O.order[take] : take - sim_lo);
                     const DevParams &Q = sims[sim];
                     T = injection_energy(Q, pid);
                     ++my_particles;
@@ -518,6 +520,71 @@ __global__ void __launch_bounds__(kBlockThreads) order_scatter_kernel(const DevP
     }
 }
 
+// Batch launches: the same order per simulation over the virtual id space.
+// Counters per simulation and bucket are global (kOrderSimBuckets per
+// simulation); the offsets sit in LDS as in batch_kernel.
+__device__ __forceinline__ uint32_t sim_of(const uint64_t *off, uint32_t n_sims, uint64_t v) {
+    uint32_t lo = 0, hi = n_sims;   // largest s with off[s] <= v
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= v) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(kBlockThreads) batch_order_count_kernel(const DevParams *__restrict__ sims,
+                                                                         uint32_t n_sims,
+                                                                         const uint64_t *__restrict__ offsets,
+                                                                         uint32_t *counts) {
+    __shared__ uint64_t off[kBatchMaxSims + 1];
+    for (uint32_t i = threadIdx.x; i <= n_sims; i += blockDim.x) off[i] = offsets[i];
+    __syncthreads();
+    const uint64_t n = off[n_sims];
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t sm = sim_of(off, n_sims, v);
+        const DevParams &P = sims[sm];
+        if (P.order.nb == 0) continue;
+        atomicAdd(&counts[(size_t)sm * kOrderSimBuckets + order_bucket(P, v - off[sm], P.order)], 1u);
+    }
+}
+
+// Workgroup s: exclusive scan of simulation s's bucket counts into cursor.
+__global__ void __launch_bounds__(kOrderSimBuckets) batch_order_scan_kernel(const DevParams *__restrict__ sims,
+                                                                          const uint32_t *counts, uint32_t *cursor) {
+    __shared__ uint32_t v[kOrderSimBuckets];
+    const uint32_t sm = blockIdx.x, t = threadIdx.x;
+    const uint32_t nb = sims[sm].order.nb;
+    const uint32_t *c = counts + (size_t)sm * kOrderSimBuckets;
+    v[t] = t < nb ? c[t] : 0u;
+    __syncthreads();
+    for (uint32_t d = 1; d < kOrderSimBuckets; d <<= 1) {
+        const uint32_t add = t >= d ? v[t - d] : 0u;
+        __syncthreads();
+        v[t] += add;
+        __syncthreads();
+    }
+    if (t < nb) cursor[(size_t)sm * kOrderSimBuckets + t] = v[t] - c[t];
+}
+
+__global__ void __launch_bounds__(kBlockThreads) batch_order_scatter_kernel(const DevParams *__restrict__ sims,
+                                                                           uint32_t n_sims,
+                                                                           const uint64_t *__restrict__ offsets,
+                                                                           uint32_t *cursor, uint32_t *perm) {
+    __shared__ uint64_t off[kBatchMaxSims + 1];
+    for (uint32_t i = threadIdx.x; i <= n_sims; i += blockDim.x) off[i] = offsets[i];
+    __syncthreads();
+    const uint64_t n = off[n_sims];
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t sm = sim_of(off, n_sims, v);
+        const DevParams &P = sims[sm];
+        const uint64_t local = v - off[sm];
+        uint64_t pos = local;
+        if (P.order.nb != 0)
+            pos = atomicAdd(&cursor[(size_t)sm * kOrderSimBuckets + order_bucket(P, local, P.order)], 1u);
+        perm[off[sm] + pos] = (uint32_t)local;
+    }
+}
+
 __global__ void zero_kernel(uint64_t *p, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
@@ -618,6 +685,23 @@ hipError_t gs_launch_order(const DevParams *sims, uint64_t n, const OrderKey &ke
     hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(kOrderMaxBuckets), 0, s, counts, cursor, key.nb);
     hipLaunchKernelGGL(order_scatter_kernel, dim3((unsigned)((n + kOrderTile - 1) / kOrderTile)), dim3(kBlockThreads),
                        0, s, sims, n, key, cursor, perm);
+    return hipGetLastError();
+}
+
+hipError_t gs_launch_batch_order(const DevParams *sims, uint32_t n_sims, const uint64_t *offsets, uint64_t n,
+                                 uint32_t *counts, uint32_t *perm, hipStream_t s) {
+    if (n_sims == 0 || n == 0) return hipSuccess;
+    if (n_sims > (uint32_t)kBatchMaxSims || ((uintptr_t)counts & 7u)) return hipErrorInvalidValue;
+    uint32_t *cursor = counts + (size_t)n_sims * kOrderSimBuckets;
+    const uint64_t words = (uint64_t)n_sims * kOrderSimBuckets / 2;   // counts, as 64-bit words
+    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)std::min<uint64_t>((words + 255) / 256, 1024)), dim3(256), 0, s,
+                       (uint64_t *)counts, (size_t)words);
+    const uint64_t blocks = std::min<uint64_t>((n + kBlockThreads - 1) / kBlockThreads, 8192);
+    hipLaunchKernelGGL(batch_order_count_kernel, dim3((unsigned)blocks), dim3(kBlockThreads), 0, s, sims, n_sims,
+                       offsets, counts);
+    hipLaunchKernelGGL(batch_order_scan_kernel, dim3(n_sims), dim3(kOrderSimBuckets), 0, s, sims, counts, cursor);
+    hipLaunchKernelGGL(batch_order_scatter_kernel, dim3((unsigned)blocks), dim3(kBlockThreads), 0, s, sims, n_sims,
+                       offsets, cursor, perm);
     return hipGetLastError();
 }
 
