@@ -192,7 +192,7 @@ def main():
             s, _, _ = eng.run_plain(q)
             st = eng.last_stats()
             util = st.active_cycles / st.lane_cycles if st.lane_cycles else 0.0
-            res.append((p.model, s.total_steps, st.kernel_ms, s, util))
+            res.append((p.model, s.total_steps, st.kernel_ms, s, util, st.max_particle_steps))
         return res
 
     for i in range(a.warmup):
@@ -241,6 +241,7 @@ def main():
             "spectra_per_s": len(models) * a.steps / wall,
             "lane_utilisation": {abi.MODEL_NAMES[m]: sum(r[4] for r in runs if r[0] == m) / a.steps
                                  for m in models},
+            "longest_particle_steps": {abi.MODEL_NAMES[m]: max(r[5] for r in runs if r[0] == m) for m in models},
         },
     }
     if a.batch_sims > 0:

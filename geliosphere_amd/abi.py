@@ -112,6 +112,7 @@ class GsRunStats(ctypes.Structure):
         ("active_cycles", ctypes.c_uint64),
         ("grid_blocks", ctypes.c_uint32),
         ("block_threads", ctypes.c_uint32),
+        ("max_particle_steps", ctypes.c_uint64),
     ]
 
 

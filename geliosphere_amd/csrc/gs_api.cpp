@@ -64,6 +64,8 @@ struct gs_ctx {
     uint64_t cap_border = 0;
     uint32_t *d_bcounts = nullptr;       // batches: 2 * kOrderSimBuckets words per simulation
     uint64_t cap_bcounts = 0;
+    gs::SpillRec *d_spill = nullptr;     // plain runs: two halves of cap_spill records
+    uint64_t cap_spill = 0;
     uint32_t block_override[4] = {0, 0, 0, 0};
     uint32_t bpcu_override[4] = {0, 0, 0, 0};
     gs_run_stats stats{};
@@ -189,6 +191,14 @@ gs_status ensure_order(gs_ctx *ctx, uint64_t n) {
     if (ctx->d_order) { (void)hipFree(ctx->d_order); ctx->d_order = nullptr; ctx->cap_order = 0; }
     HIP_TRY(ctx, hipMalloc((void **)&ctx->d_order, n * sizeof(uint32_t)));
     ctx->cap_order = n;
+    return GS_OK;
+}
+
+gs_status ensure_spill(gs_ctx *ctx, uint64_t n) {
+    if (ctx->cap_spill >= n) return GS_OK;
+    if (ctx->d_spill) { (void)hipFree(ctx->d_spill); ctx->d_spill = nullptr; ctx->cap_spill = 0; }
+    HIP_TRY(ctx, hipMalloc((void **)&ctx->d_spill, 2 * n * sizeof(gs::SpillRec)));
+    ctx->cap_spill = n;
     return GS_OK;
 }
 
@@ -331,6 +341,13 @@ unsigned fold_blocks(uint64_t events) {
 // produce at most one event per particle) to 2^27 x 64 B = 8.6 GB.
 const uint64_t kMaxChunk = 1ull << 27;
 const uint64_t kOrderMinParticles = 4096;   // smaller runs keep id order
+// Tail compaction of plain runs (DevOut::spill): used for 1D F-p, whose
+// lifetimes are heavy-tailed with no energy trend, and for runs of at most
+// kCompactPerLane particles per lane; continuations stop handing on below
+// kCompactMin particles, and after kCompactRounds rounds.
+const uint64_t kCompactPerLane = 16;
+const uint64_t kCompactMin = 1024;
+const int kCompactRounds = 48;
 
 uint64_t event_room(int model, uint64_t n) {
     if (model == GS_MODEL_1D_FP) return std::max<uint64_t>(n / 16, 1u << 16);
@@ -497,6 +514,8 @@ gs_status run_sub_batch(gs_ctx *ctx, const gs_params *sims, const std::vector<ui
             O.queue = ctx->d_meta + q_base + l;
             O.cycles = ctx->d_meta + c_base + l;
             O.order = nullptr;
+            O.spill = nullptr; O.spill_count = nullptr; O.spill_cap = 0;
+            O.resume = nullptr; O.resume_n = 0;
             if (ordered) {
                 e = gs_launch_batch_order(ctx->d_params + L.base, L.count, ctx->d_meta + L.off_pos, nv[l],
                                           ctx->d_bcounts + 2 * (size_t)gs::kOrderSimBuckets * L.base,
@@ -522,6 +541,8 @@ gs_status run_sub_batch(gs_ctx *ctx, const gs_params *sims, const std::vector<ui
         if (e == hipSuccess) {
             DevOut O;
             O.queue = nullptr; O.cycles = nullptr; O.order = nullptr;
+            O.spill = nullptr; O.spill_count = nullptr; O.spill_cap = 0;
+            O.resume = nullptr; O.resume_n = 0;
             O.g_count = W.count; O.g_digits = W.digits; O.g_stats = W.stats;
             O.events = ctx->d_events;
             O.ev_off = ctx->d_meta; O.ev_cap = ctx->d_meta + m; O.n_events = ctx->d_meta + 2 * m;
@@ -617,6 +638,7 @@ void gs_destroy(gs_ctx *ctx) {
     if (ctx->d_order) (void)hipFree(ctx->d_order);
     if (ctx->d_order_counts) (void)hipFree(ctx->d_order_counts);
     if (ctx->d_border) (void)hipFree(ctx->d_border);
+    if (ctx->d_spill) (void)hipFree(ctx->d_spill);
     if (ctx->d_bcounts) (void)hipFree(ctx->d_bcounts);
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     for (int i = 0; i < 3; ++i) {
@@ -748,6 +770,9 @@ gs_status gs_run_plain(gs_ctx *ctx, const gs_params *p, gs_spectrum *out, gs_exi
     O.ev_off = ctx->d_meta; O.ev_cap = ctx->d_meta + 1; O.n_events = ctx->d_meta + 2;
     O.cycles = ctx->d_meta + 3;
     O.order = nullptr;
+    O.spill = nullptr; O.spill_count = nullptr; O.spill_cap = 0;
+    O.resume = nullptr; O.resume_n = 0;
+    const uint64_t lanes = (uint64_t)grid * gs::kBlockThreads;
     ctx->h_meta.assign(2, 0);
     for (uint64_t c0 = 0; c0 < N || (N == 0 && c0 == 0); c0 += kMaxChunk) {
         gs_params q = *p;
@@ -774,10 +799,42 @@ gs_status gs_run_plain(gs_ctx *ctx, const gs_params *p, gs_spectrum *out, gs_exi
             HIP_TRY(ctx, hipMemcpyAsync(ctx->d_meta, ctx->h_meta.data(), 2 * sizeof(uint64_t),
                                         hipMemcpyHostToDevice, ctx->stream));
             HIP_TRY(ctx, gs_launch_zero(L.queue, 1, ctx->stream));
-            HIP_TRY(ctx, gs_launch_zero(ctx->d_meta + 2, 2, ctx->stream));
+            HIP_TRY(ctx, gs_launch_zero(ctx->d_meta + 2, 4, ctx->stream));   // events, cycles, spill counts
             O.events = ctx->d_events;
+            const bool compact = q.model == GS_MODEL_1D_FP || q.n_particles <= kCompactPerLane * lanes;
+            if (compact) {
+                st = ensure_spill(ctx, lanes);   // at most one hand-on per lane and launch
+                if (st != GS_OK) return st;
+            }
+            O.resume = nullptr;
+            O.resume_n = 0;
+            O.spill = compact ? ctx->d_spill : nullptr;
+            O.spill_count = ctx->d_meta + 4;
+            O.spill_cap = ctx->cap_spill;
             HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
             HIP_TRY(ctx, gs_launch_traj(q.model, false, ctx->d_params, 1, nullptr, O, grid, ctx->stream));
+            // continuations: the handed-on particles, packed into full waves
+            for (int round = 0, cur = 0; compact; ++round, cur ^= 1) {
+                uint64_t pending = 0;
+                HIP_TRY(ctx, hipMemcpyAsync(&pending, ctx->d_meta + 4 + cur, sizeof pending, hipMemcpyDeviceToHost,
+                                            ctx->stream));
+                HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+                pending = std::min(pending, O.spill_cap);   // slots beyond the cap were not handed on
+                if (pending == 0) break;
+                HIP_TRY(ctx, gs_launch_zero(L.queue, 1, ctx->stream));
+                HIP_TRY(ctx, gs_launch_zero(ctx->d_meta + 4 + (cur ^ 1), 1, ctx->stream));
+                O.resume = ctx->d_spill + (size_t)cur * ctx->cap_spill;
+                O.resume_n = pending;
+                O.spill = pending > kCompactMin && round + 1 < kCompactRounds
+                              ? ctx->d_spill + (size_t)(cur ^ 1) * ctx->cap_spill : nullptr;
+                O.spill_count = ctx->d_meta + 4 + (cur ^ 1);
+                O.spill_cap = ctx->cap_spill;
+                const unsigned g2 = (unsigned)std::min<uint64_t>(grid, (pending + gs::kBlockThreads - 1) /
+                                                                          gs::kBlockThreads);
+                HIP_TRY(ctx, gs_launch_traj(q.model, false, ctx->d_params, 1, nullptr, O, g2, ctx->stream));
+            }
+            O.resume = nullptr;
+            O.spill = nullptr;
             HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
             HIP_TRY(ctx, hipMemcpyAsync(&nev, ctx->d_meta + 2, sizeof nev, hipMemcpyDeviceToHost, ctx->stream));
             HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -820,6 +877,7 @@ gs_status gs_run_plain(gs_ctx *ctx, const gs_params *p, gs_spectrum *out, gs_exi
     ctx->stats.kernel_ms = kernel_ms;
     ctx->stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     ctx->stats.lane_cycles = H.stats[gs::ST_CYCLES] * (uint64_t)gs::kWave * gs::kCycleSteps;
+    ctx->stats.max_particle_steps = H.stats[gs::ST_MAX_STEPS];
     ctx->stats.active_cycles = H.stats[gs::ST_STEPS];
     ctx->stats.grid_blocks = grid;
     ctx->stats.block_threads = gs::kBlockThreads;

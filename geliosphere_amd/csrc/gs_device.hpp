@@ -287,6 +287,8 @@ struct Bp {
     static constexpr int kNormals = 1;
     static constexpr double kE2Exit = (200.0 + kT0) * (200.0 + kT0);
     double r, P, beta, t_inj;
+    __device__ __forceinline__ void save(double *s) const { s[0] = r; s[1] = P; s[2] = beta; s[3] = t_inj; }
+    __device__ __forceinline__ void load(const double *s) { r = s[0]; P = s[1]; beta = s[2]; t_inj = s[3]; }
     __device__ __forceinline__ bool init(const StepParams &Pm, double T) {
         t_inj = T;
         P = fm::sqrt_nn(T * (T + 2.0 * kT0));
@@ -334,6 +336,12 @@ struct Fp {
     static constexpr int kNormals = 1;
     static constexpr double kE2Cross = (100.0 + kT0) * (100.0 + kT0);
     double r, P, beta, R, sumac, t_inj;
+    __device__ __forceinline__ void save(double *s) const {
+        s[0] = r; s[1] = P; s[2] = beta; s[3] = R; s[4] = sumac; s[5] = t_inj;
+    }
+    __device__ __forceinline__ void load(const double *s) {
+        r = s[0]; P = s[1]; beta = s[2]; R = s[3]; sumac = s[4]; t_inj = s[5];
+    }
     __device__ __forceinline__ bool init(const StepParams &Pm, double T) {
         t_inj = T;
         P = fm::sqrt_nn(T * (T + 2.0 * kT0));
@@ -378,6 +386,8 @@ struct Fp {
 struct Sp {
     static constexpr int kNormals = 2;
     double r, theta, T, t_inj;
+    __device__ __forceinline__ void save(double *s) const { s[0] = r; s[1] = theta; s[2] = T; s[3] = t_inj; }
+    __device__ __forceinline__ void load(const double *s) { r = s[0]; theta = s[1]; T = s[2]; t_inj = s[3]; }
     __device__ __forceinline__ bool init(const StepParams &Pm, double Tin) {
         t_inj = Tin;
         T = Tin;
@@ -454,6 +464,8 @@ struct Sp {
 struct Ge {
     static constexpr int kNormals = 3;
     double r, theta, T, t_inj;
+    __device__ __forceinline__ void save(double *s) const { s[0] = r; s[1] = theta; s[2] = T; s[3] = t_inj; }
+    __device__ __forceinline__ void load(const double *s) { r = s[0]; theta = s[1]; T = s[2]; t_inj = s[3]; }
     __device__ __forceinline__ bool init(const StepParams &Pm, double Tin) {
         t_inj = Tin;
         T = Tin;

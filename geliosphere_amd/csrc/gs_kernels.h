@@ -26,6 +26,7 @@ enum Stat {
     ST_OVERFLOW,
     ST_CYCLES,    // wave cycles issued (x 64 x kCycleSteps = lane-step slots)
     ST_NREC,      // events offered to the record buffer
+    ST_MAX_STEPS, // plain runs: most steps of one particle (a maximum, not a sum)
     ST_COUNT_
 };
 
@@ -82,6 +83,18 @@ struct alignas(64) DevEvent {
 };
 static_assert(sizeof(DevEvent) == 64, "DevEvent is one 64-byte line");
 
+// A live particle handed from one plain launch to the next (see DevOut):
+// the model's state, its id, steps taken and next Philox block.  The
+// counter-based normal stream makes the continuation exact.
+struct alignas(64) SpillRec {
+    double s[6];
+    uint64_t pid;
+    uint32_t steps, blk;
+};
+static_assert(sizeof(SpillRec) == 64, "SpillRec is one 64-byte line");
+constexpr int kSpillLanes = 16;      // a wave at or below this many live lanes hands them on
+constexpr uint32_t kSpillMinCycles = 4;   // after at least this many cycles in its launch
+
 // Device buffers of a launch over n_sims simulations (plain runs: 1).
 // Per simulation s: histogram count[GS_MAX_BINS], digits[GS_MAX_WEIGHTS][2]
 // [GS_MAX_BINS][GS_ACC_LIMBS] (32-bit values in 64-bit words), stats
@@ -98,6 +111,17 @@ struct DevOut {
     uint64_t *cycles;         // wave cycles issued by the launch
     const uint32_t *order;    // queue position (plain) or virtual id (batch) -> particle offset
                               // within its simulation (nullptr: identity)
+    // Plain runs only.  Tail compaction: once the queue is exhausted, a wave
+    // with at most kSpillLanes live lanes writes them to spill[*spill_count++]
+    // and exits (spill == nullptr: never).  Slots at or beyond spill_cap are
+    // not written: such particles go on in place.  A continuation launch
+    // takes its particles from resume[0 .. resume_n) instead of the queue of
+    // new ids.
+    SpillRec *spill;
+    uint64_t *spill_count;
+    uint64_t spill_cap;
+    const SpillRec *resume;
+    uint64_t resume_n;
 };
 
 // LDS layout per workgroup: count[n_bins] | digits[n_weights][2][n_bins][16] | stat[ST_COUNT_]

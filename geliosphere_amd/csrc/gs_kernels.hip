@@ -125,7 +125,8 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
     const int lane = (int)(threadIdx.x & (kWave - 1));
     const uint64_t lane_lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const DevParams &P0 = sims[0];
-    const uint64_t N = kBatch ? off[n_sims] : P0.n_particles;
+    const bool resume = !kBatch && O.resume != nullptr;
+    const uint64_t N = kBatch ? off[n_sims] : (resume ? O.resume_n : P0.n_particles);
 
     // Per-simulation values (wave-uniform for plain runs, per lane in batches).
     uint32_t k0 = (uint32_t)P0.seed, k1 = (uint32_t)(P0.seed >> 32);
@@ -143,7 +144,7 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
     uint64_t q_next = 0, q_end = 0;
     bool exhausted = false;
     uint64_t my_steps = 0;
-    uint32_t my_particles = 0, my_nan = 0, my_cap = 0, cycles = 0;
+    uint32_t my_particles = 0, my_nan = 0, my_cap = 0, cycles = 0, my_max = 0;
 
     auto emit = [&](double t_inj, double t_exit, double r, double theta, double sumac) {
         if constexpr (kBatch) emit_event(O, pid, sim, t_inj, t_exit, r, theta, sumac);
@@ -188,6 +189,7 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
                 // it stopped on a NaN state
                 if (has_particle) {
                     my_steps += steps;
+                    my_max = steps > my_max ? steps : my_max;
                     if (isnan(m.r)) ++my_nan;
                 }
                 has_particle = true;
@@ -220,6 +222,14 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
                     T = injection_energy(Q, pid);
                     ++my_particles;
                     active = m.init(lp, T);
+                } else if (resume) {
+                    // a particle handed on by the previous launch
+                    const SpillRec &rec = O.resume[take];
+                    m.load(rec.s);
+                    pid = rec.pid;
+                    steps = rec.steps;
+                    blk = rec.blk;
+                    active = true;
                 } else {
                     pid = first + (O.order ? (uint64_t)O.order[take] : take);
                     T = injection_energy(P0, pid);
@@ -228,7 +238,29 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
                 }
             }
         }
-        if constexpr (!kBatch) stage_flush(stage, O, kStageFlush);
+        if constexpr (!kBatch) {
+            // tail compaction: a sparse wave hands its live particles on and exits
+            if (O.spill && exhausted && cycles >= kSpillMinCycles) {
+                const uint64_t live = __ballot(active);
+                const int n_live = __popcll(live);
+                if (n_live > 0 && n_live <= kSpillLanes) {
+                    unsigned long long base = 0;
+                    if (lane == 0) base = atomicAdd((unsigned long long *)O.spill_count, (unsigned long long)n_live);
+                    base = __shfl(base, 0);
+                    const uint64_t slot = base + (uint64_t)__popcll(live & lane_lt);
+                    if (active && slot < O.spill_cap) {
+                        SpillRec &rec = O.spill[slot];
+                        m.save(rec.s);
+                        rec.pid = pid;
+                        rec.steps = steps;
+                        rec.blk = blk;
+                        active = false;
+                        has_particle = false;   // its steps are counted where it ends
+                    }
+                }
+            }
+            stage_flush(stage, O, kStageFlush);
+        }
         if (__ballot(active) == 0) {
             if (exhausted) break;
             continue;
@@ -301,6 +333,7 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
     }
     if (has_particle) {
         my_steps += steps;
+        my_max = steps > my_max ? steps : my_max;
         if (isnan(m.r)) ++my_nan;
     }
     if constexpr (!kBatch) stage_flush(stage, O, 1);
@@ -313,6 +346,7 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
         if (my_nan) atomicAdd((unsigned long long *)&stat[ST_NAN], (unsigned long long)my_nan);
         if (my_cap) atomicAdd((unsigned long long *)&stat[ST_CAP], (unsigned long long)my_cap);
         if (lane == 0) atomicAdd((unsigned long long *)&stat[ST_CYCLES], (unsigned long long)cycles);
+        if (my_max) atomicMax((unsigned long long *)&stat[ST_MAX_STEPS], (unsigned long long)my_max);
     }
 }
 
@@ -363,8 +397,11 @@ plain_kernel(const DevParams *__restrict__ Pp, const DevOut O) {
     __syncthreads();
     run_lanes<Model, false>(Pp, 1, nullptr, stat, &stage, bm, O);
     __syncthreads();
-    if (threadIdx.x < ST_COUNT_ && stat[threadIdx.x])
-        atomicAdd((unsigned long long *)&O.g_stats[threadIdx.x], (unsigned long long)stat[threadIdx.x]);
+    if (threadIdx.x < ST_COUNT_ && stat[threadIdx.x]) {
+        unsigned long long *g = (unsigned long long *)&O.g_stats[threadIdx.x];
+        if (threadIdx.x == ST_MAX_STEPS) atomicMax(g, (unsigned long long)stat[threadIdx.x]);
+        else atomicAdd(g, (unsigned long long)stat[threadIdx.x]);
+    }
 }
 
 // Batch of n_sims simulations of one model (n_sims <= kBatchMaxSims).

@@ -167,6 +167,7 @@ typedef struct gs_run_stats {
     uint64_t active_cycles;  /* lane-cycles with a live particle             */
     uint32_t grid_blocks;
     uint32_t block_threads;
+    uint64_t max_particle_steps; /* plain runs: most steps of one particle     */
 } gs_run_stats;
 
 typedef struct gs_ctx gs_ctx;

@@ -80,6 +80,7 @@ def same_seed_check(engine, oracle, p, label):
     say("same-seed parity", label, "N", n)
     t = time.time()
     gs, grecs, gn = engine.run_plain(p, records=True, rec_cap=4 * n + 64)
+    longest = engine.last_stats().max_particle_steps
     tg = time.time() - t
     t = time.time()
     rc, os_, orecs, on = oracle.run(p, n_threads=0, records=True, rec_cap=4 * n + 64)
@@ -91,6 +92,8 @@ def same_seed_check(engine, oracle, p, label):
     say("  steps gpu %d oracle %d" % (gs.total_steps, os_.total_steps))
     if model in (abi.MODEL_1D_BP, abi.MODEL_1D_FP):
         assert fg[2] == 1.0 and gn == on and gs.total_steps == os_.total_steps
+        if n <= 4000:   # the longest trajectory, particle by particle on the oracle
+            assert longest == max(oracle.particle(p, p.first_particle + i)[1] for i in range(n))
         for b in range(p.n_bins):
             assert gs.count[b] == os_.count[b]
             for k in range(p.n_weights):
