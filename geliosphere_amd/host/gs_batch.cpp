@@ -121,6 +121,17 @@ int run_batch_rows(ParamsCarrier *pc, const std::vector<BatchRow> &rows) {
             std::fprintf(stderr, "[error] V is out of range!(100-1500 km/s); skipping simulation %zu.\n", i + 1);
             continue;
         }
+        // BatchRun.cpp:262-273
+        if ((b.month != -1) != (b.year != -1)) {
+            std::fprintf(stderr, "[error] Both month and year must be set; skipping simulation %zu.\n", i + 1);
+            continue;
+        }
+        const bool dated = b.month != -1 && b.year != -1;
+        if (dated && (b.K0 != -1.0 || b.V != -1.0)) {
+            std::fprintf(stderr, "[error] Simulation dates cannot be combined with K0 or V; skipping simulation %zu.\n",
+                         i + 1);
+            continue;
+        }
         pc->putString("model", b.model);
         if (b.dt != -1.0) pc->putFloat("dt", (float)b.dt);
         if (b.K0 != -1.0) {
@@ -132,6 +143,11 @@ int run_batch_rows(ParamsCarrier *pc, const std::vector<BatchRow> &rows) {
         if (b.r != -1.0) pc->putFloat("r_injection", (float)b.r);
         if (b.theta != -1.0) pc->putFloat("theta_injection", (float)b.theta);
         if (!b.name.empty() && b.name.find_first_not_of(' ') != std::string::npos) pc->putString("destination", b.name);
+        if (dated) {   // BatchRun.cpp:337-340, after the row's other values
+            std::string terr;
+            if (!month_year_check(pc, model, b.year, b.month, pc->getString("currentApplicationPath", "./"), &terr))
+                std::fprintf(stderr, "[error] %s (simulation %zu)\n", terr.c_str(), i + 1);
+        }
         // Distinct particle streams per row unless the caller fixed a seed.
         if (!seed0.empty()) pc->putString("gs_seed", std::to_string(std::strtoull(seed0.c_str(), nullptr, 10) + i));
         RunConfig cfg;

@@ -7,12 +7,14 @@
 // Algorithm step: simulation, Results pass, removal of log.dat.
 //
 //   gs_cli (-F|-B|-E|-T|-b FILE) [-N millions] [-K K0] [-V V] [-d dt]
-//          [-p destination] [-c] [--evaluation log.dat]
+//          [-m month -y year] [-p destination] [-c] [--evaluation log.dat]
 //          [--r AU] [--theta deg] [--tilt deg] [--polarity +-1]
 //          [--seed N] [--particles N] [--keep-log] [--no-log]
 //          [--solarprop-gpu-compat] [--device N] [--print-params]
 //
-// Defaults are those of the reference's shipped Settings.toml.
+// Defaults are those of the reference's shipped Settings.toml.  -m/-y take
+// K0 and V (1D) or K0, tilt and polarity (2D) from the measured-value table
+// next to the executable; they exclude -K and -V, and -m needs -y.
 // --print-params prints the parameter block the model would run with (JSON)
 // and exits without touching a device.
 #include <cstdio>
@@ -46,7 +48,8 @@ void put_settings_defaults(ParamsCarrier *pc) {
 void usage() {
     std::fprintf(stderr,
                  "usage: gs_cli (-F|-B|-E|-T|-b FILE) [-N millions] [-K K0_cm2s] [-V V_kms] [-d dt]\n"
-                 "              [-p destination] [-c] [--evaluation LOG] [--r AU] [--theta deg]\n"
+                 "              [-m month -y year] [-p destination] [-c] [--evaluation LOG]\n"
+                 "              [--r AU] [--theta deg]\n"
                  "              [--tilt deg] [--polarity 1|-1] [--seed N] [--particles N]\n"
                  "              [--keep-log] [--no-log] [--solarprop-gpu-compat] [--device N]\n"
                  "              [--print-params]\n");
@@ -59,6 +62,12 @@ int main(int argc, char **argv) {
     int model = -1;
     std::string batch;
     bool print_params = false;
+    int month = -1, year = -1;
+    bool k_set = false, v_set = false, other_set = false;
+    // the application directory: argv[0] up to its last '/' (ParseParams.cpp getApplicationPath)
+    const std::string a0 = argv[0];
+    const std::string app_dir = a0.rfind('/') == std::string::npos ? "" : a0.substr(0, a0.rfind('/') + 1);
+    pc->putString("currentApplicationPath", app_dir);
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         auto val = [&]() -> const char * {
@@ -78,19 +87,27 @@ int main(int argc, char **argv) {
             const int n = std::atoi(val());
             if (n <= 0) { std::fprintf(stderr, "Number of test particles must be greater than 0!\n"); return 2; }
             pc->putInt("millions", n);
+            other_set = true;
+        } else if (a == "-m" || a == "--month") {
+            month = std::atoi(val());
+        } else if (a == "-y" || a == "--year") {
+            year = std::atoi(val());
         } else if (a == "-K" || a == "--K0") {
             const float K = (float)std::atof(val());
             if (K < 0.0f) { std::fprintf(stderr, "K0 is out of range!(>0)\n"); return 2; }
             pc->putFloat("K0", (float)(K * 4.4683705e-27));
             pc->putInt("K0_entered_by_user", 1);
+            k_set = true;
         } else if (a == "-V" || a == "--V") {
             const float V = (float)std::atof(val());
             if (V < 100.0f || V > 1500.0f) { std::fprintf(stderr, "V is out of range!(100-1500 km/s)\n"); return 2; }
             pc->putFloat("V", (float)(V * 6.68458712e-9));
+            v_set = true;
         } else if (a == "-d" || a == "--dt") {
             const float dt = (float)std::atof(val());
             if (dt < 3.0f || dt > 5000.0f) { std::fprintf(stderr, "dt is out of range!(3-5000)\n"); return 2; }
             pc->putFloat("dt", dt);
+            other_set = true;
         } else if (a == "-p" || a == "--path") pc->putString("destination", val());
         else if (a == "--evaluation") {
             pc->putInt("run_simulation", 0);
@@ -111,9 +128,17 @@ int main(int argc, char **argv) {
             return 2;
         }
     }
+    // option relations of the reference CLI (ParseParams.cpp:49-62)
+    const bool date = month != -1 || year != -1;
+    if ((k_set || v_set) && date) { std::fprintf(stderr, "-K/-V exclude -m/-y\n"); return 2; }
+    if (month != -1 && year == -1) { std::fprintf(stderr, "-m requires -y\n"); return 2; }
     put_settings_defaults(pc);
     if (!batch.empty()) {
         if (model >= 0) { std::fprintf(stderr, "-b excludes the model flags\n"); return 2; }
+        if (k_set || v_set || other_set || date) {
+            std::fprintf(stderr, "-b excludes -d, -N, -K, -V, -m and -y\n");
+            return 2;
+        }
         std::vector<gsdrop::BatchRow> rows;
         std::string err;
         if (!gsdrop::read_batch_csv(batch, &rows, &err)) {
@@ -137,6 +162,11 @@ int main(int argc, char **argv) {
         return 2;
     }
     pc->putString("model", gsdrop::model_name(model));
+    if (month != -1 && year != -1) {
+        std::string err;
+        if (!gsdrop::month_year_check(pc, model, year, month, app_dir, &err))
+            std::fprintf(stderr, "[error] %s\n", err.c_str());   // the run goes on without the table values
+    }
     if (print_params) {
         gsdrop::RunConfig cfg;
         std::string err;

@@ -107,6 +107,20 @@ bool read_batch_csv(const std::string &path, std::vector<BatchRow> *rows, std::s
 // the number of rows run.
 int run_batch_rows(ParamsCarrier *pc, const std::vector<BatchRow> &rows);
 
+// Measured values for a month and year (Input/src/MeasureValuesTransformation.cpp,
+// InputValidation.cpp:22-37,130-151): the table file of `model`, the row key
+// (1D: "<year>.<month code>", 2D: the Carrington rotation of the 15th of the
+// month), and the lookup that puts K0 and V (1D) or K0, tilt_angle and
+// polarity (2D) into the carrier, with month_option / year_option.  `dir` is
+// the application directory (ending in '/').  False, with the reason in
+// *err, when the file, a column or the row is missing.
+const char *table_name(int model);
+long carrington_rotation(int month, int year);
+std::string table_row_id(int model, int month, int year);
+bool table_lookup(const std::string &path, const std::string &row_id, const std::vector<std::string> &columns,
+                  std::vector<std::string> *values, std::string *err);
+bool month_year_check(ParamsCarrier *pc, int model, int year, int month, const std::string &dir, std::string *err);
+
 }  // namespace gsdrop
 
 #endif

@@ -191,3 +191,76 @@ def test_results_2d_files(tmp_path):
     assert sum(r[3] for r in weber) == 2.0        # the last line (index 10) is not read
     uly = read_rows(tmp_path / "Ulysses.dat")
     assert uly[0][3] == 1.0 and uly[1][3] == 1.0   # 0.2 GeV in (0.125, 0.25); 1.0 in (0.25, 2)
+
+
+# ---- measured-value tables (-m / -y)
+
+TABLES = os.path.join(ROOT, "tests", "golden", "k0_tables")   # rows excerpted from the reference's CSV files
+
+
+def carrington(month, year):
+    """The reference's rotation number of the 15th of a month
+    (MeasureValuesTransformation.cpp:80-115), restated with datetime."""
+    import datetime
+    s = (datetime.datetime(year, month, 15) - datetime.datetime(1976, 5, 27)).total_seconds()
+    r = s / 2356560.0
+    frac = math.modf(r)[0]
+    return int(math.trunc(r) if frac <= 0.5499 else math.ceil(r)) + 1642
+
+
+def app_with_tables(tmp_path):
+    """gs_cli started from a directory that holds the tables: the reference
+    reads them next to its executable (ParseParams.cpp getApplicationPath)."""
+    for f in os.listdir(TABLES):
+        (tmp_path / f).write_bytes(open(os.path.join(TABLES, f), "rb").read())
+    link = tmp_path / "gs_cli"
+    if not os.path.exists(CLI):
+        pytest.fail("gs_cli is not built; run __graft_entry__.build()")
+    os.symlink(CLI, link)
+    return str(link)
+
+
+def run_app(app, *args, check=True):
+    r = subprocess.run([app, *args], capture_output=True, text=True, timeout=120)
+    if check and r.returncode != 0:
+        raise AssertionError("gs_cli %s failed (%d): %s" % (" ".join(args), r.returncode, r.stderr))
+    return r
+
+
+def test_carrington_rotation_rule():
+    # the table rows: rotation 1642 begins on 1976-05-27, 1643 on 06-23, 2000 on 2003-02-20
+    assert carrington(5, 1976) == 1642 and carrington(6, 1976) == 1643 and carrington(8, 1976) == 1645
+    assert carrington(2, 2003) == 2000
+
+
+def test_month_year_1d_takes_v_from_the_table(tmp_path):
+    app = app_with_tables(tmp_path)
+    p = json.loads(run_app(app, "-B", "-m", "6", "-y", "1990", "--seed", "1", "--print-params").stdout)
+    assert p["V"] == conv_v(435)          # row 1990.45: V = 435 km/s
+    # the 1D GPU models keep their built-in K0 unless it was entered (CosmicConstants.cu:69-73)
+    assert p["K0"] == float(f32(0.000222))
+    p = json.loads(run_app(app, "-F", "-m", "12", "-y", "2008", "--seed", "1", "--print-params").stdout)
+    assert p["V"] == conv_v(380)          # row 2008.95
+
+
+def test_month_year_2d_takes_k0_tilt_polarity(tmp_path):
+    app = app_with_tables(tmp_path)
+    g = json.loads(run_app(app, "-T", "-m", "6", "-y", "1976", "--seed", "1", "--print-params").stdout)
+    assert g["K0"] == float(f32(f32(5.47252112676056e-05) * f32(1.0)))   # rotation 1643, x K0_ratio
+    assert g["tilt"] == deg_f32(6.2) and g["polarity"] == 1.0
+    e = json.loads(run_app(app, "-E", "-m", "2", "-y", "2003", "--seed", "1", "--print-params").stdout)
+    assert e["K0"] == float(f32(6.94405829596413e-05))                   # SolarProp table, not scaled
+    assert e["tilt"] == deg_f32(43.8) and e["polarity"] == -1.0
+    g2 = json.loads(run_app(app, "-T", "-m", "2", "-y", "2003", "--seed", "1", "--print-params").stdout)
+    assert g2["tilt"] == deg_f32(43.8) and g2["polarity"] == -1.0       # rotation 2000, A < 0
+
+
+def test_month_year_missing_row_and_option_rules(tmp_path):
+    app = app_with_tables(tmp_path)
+    r = run_app(app, "-T", "-m", "1", "-y", "1960", "--seed", "1", "--print-params")
+    assert "not found in input table" in r.stderr      # logged; the run keeps its defaults
+    d = json.loads(r.stdout)
+    assert d["polarity"] == 1.0
+    assert run_app(app, "-B", "-m", "6", "--print-params", check=False).returncode != 0          # -m needs -y
+    assert run_app(app, "-B", "-K", "5e22", "-m", "6", "-y", "1990", "--print-params", check=False).returncode != 0
+    assert run_app(app, "-B", "-V", "400", "-y", "1990", "--print-params", check=False).returncode != 0
