@@ -140,8 +140,19 @@ int run_batch_rows(ParamsCarrier *pc, const std::vector<BatchRow> &rows) {
         }
         if (b.N != -1) pc->putInt("millions", (int)b.N);
         if (b.V != -1.0) pc->putFloat("V", (float)((float)b.V * 6.68458712e-9));
-        if (b.r != -1.0) pc->putFloat("r_injection", (float)b.r);
-        if (b.theta != -1.0) pc->putFloat("theta_injection", (float)b.theta);
+        // The row's settings file, or the batch defaults, with its r and theta
+        // (BatchRun.cpp:195-216,335-336).
+        Settings st = batch_default_settings();
+        if (!b.settings.empty() && access(b.settings.c_str(), R_OK) == 0) {
+            std::string serr;
+            if (!read_settings(b.settings, &st, &serr)) {
+                std::fprintf(stderr, "[error] %s; skipping simulation %zu.\n", serr.c_str(), i + 1);
+                continue;
+            }
+        }
+        if (b.r > 0.0) st.r_injection = b.r;
+        if (b.theta > 0.0) st.theta_injection = b.theta;
+        put_settings(pc, st);
         if (!b.name.empty() && b.name.find_first_not_of(' ') != std::string::npos) pc->putString("destination", b.name);
         if (dated) {   // BatchRun.cpp:337-340, after the row's other values
             std::string terr;

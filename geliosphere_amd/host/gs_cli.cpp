@@ -7,16 +7,21 @@
 // Algorithm step: simulation, Results pass, removal of log.dat.
 //
 //   gs_cli (-F|-B|-E|-T|-b FILE) [-N millions] [-K K0] [-V V] [-d dt]
-//          [-m month -y year] [-p destination] [-c] [--evaluation log.dat]
+//          [-m month -y year] [-s settings.toml] [-p destination] [-c]
+//          [--evaluation log.dat]
 //          [--r AU] [--theta deg] [--tilt deg] [--polarity +-1]
 //          [--seed N] [--particles N] [--keep-log] [--no-log]
 //          [--solarprop-gpu-compat] [--device N] [--print-params]
 //
-// Defaults are those of the reference's shipped Settings.toml.  -m/-y take
+// Settings come from -s FILE (the reference's TOML schema); without -s the
+// reference reads no settings file (its default-path check tests the empty
+// option string, ParseParams.cpp:126-133), so neither does gs_cli.  -m/-y take
 // K0 and V (1D) or K0, tilt and polarity (2D) from the measured-value table
 // next to the executable; they exclude -K and -V, and -m needs -y.
 // --print-params prints the parameter block the model would run with (JSON)
 // and exits without touching a device.
+#include <unistd.h>
+
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -26,29 +31,11 @@
 
 namespace {
 
-// The defaults the reference application loads from its Settings.toml
-// (Input/src/TomlSettings.cpp:13-52 reading Settings.toml); the carrier keeps
-// values already set by flags.
-void put_settings_defaults(ParamsCarrier *pc) {
-    pc->putFloat("K0_default", (float)(5e22 * 4.4683705e-27));
-    pc->putFloat("V_default", (float)(400.0 * 6.68458712e-9));
-    pc->putFloat("dt_default", 50.0f);
-    pc->putFloat("r_injection", 1.0f);
-    pc->putFloat("uniform_energy_injection_maximum", 101.0f);
-    pc->putFloat("theta_injection", 90.0f);
-    pc->putInt("use_uniform_injection", 0);
-    pc->putFloat("solarPropRatio", 0.02f);
-    pc->putFloat("geliosphereRatio", 0.2f);
-    pc->putFloat("K0_ratio", 1.0f);
-    pc->putFloat("C_delta", 8.7e-5f);
-    pc->putFloat("default_tilt_angle", 0.1f);
-    pc->putInt("remove_log_files_after_simulation", 1);
-}
-
 void usage() {
     std::fprintf(stderr,
                  "usage: gs_cli (-F|-B|-E|-T|-b FILE) [-N millions] [-K K0_cm2s] [-V V_kms] [-d dt]\n"
-                 "              [-m month -y year] [-p destination] [-c] [--evaluation LOG]\n"
+                 "              [-m month -y year] [-s settings.toml] [-p destination] [-c]\n"
+                 "              [--evaluation LOG]\n"
                  "              [--r AU] [--theta deg]\n"
                  "              [--tilt deg] [--polarity 1|-1] [--seed N] [--particles N]\n"
                  "              [--keep-log] [--no-log] [--solarprop-gpu-compat] [--device N]\n"
@@ -63,6 +50,10 @@ int main(int argc, char **argv) {
     std::string batch;
     bool print_params = false;
     int month = -1, year = -1;
+    std::string settings;
+    bool settings_set = false;
+    // The carrier keeps the first value put under a key (ParamsCarrier's map
+    // insert), so flags given here win over the settings file put below.
     bool k_set = false, v_set = false, other_set = false;
     // the application directory: argv[0] up to its last '/' (ParseParams.cpp getApplicationPath)
     const std::string a0 = argv[0];
@@ -88,6 +79,9 @@ int main(int argc, char **argv) {
             if (n <= 0) { std::fprintf(stderr, "Number of test particles must be greater than 0!\n"); return 2; }
             pc->putInt("millions", n);
             other_set = true;
+        } else if (a == "-s" || a == "--settings") {
+            settings = val();
+            settings_set = true;
         } else if (a == "-m" || a == "--month") {
             month = std::atoi(val());
         } else if (a == "-y" || a == "--year") {
@@ -132,7 +126,18 @@ int main(int argc, char **argv) {
     const bool date = month != -1 || year != -1;
     if ((k_set || v_set) && date) { std::fprintf(stderr, "-K/-V exclude -m/-y\n"); return 2; }
     if (month != -1 && year == -1) { std::fprintf(stderr, "-m requires -y\n"); return 2; }
-    put_settings_defaults(pc);
+    if (settings_set && access(settings.c_str(), F_OK) == 0) {
+        gsdrop::Settings st;
+        std::string err;
+        if (!gsdrop::read_settings(settings, &st, &err)) {
+            std::fprintf(stderr, "[error] %s\n", err.c_str());
+            return 1;
+        }
+        gsdrop::put_settings(pc, st);
+    } else {
+        std::fprintf(stderr, settings_set ? "[warning] No settings file exists on entered path.\n"
+                                          : "[warning] No settings file exists on default path.\n");
+    }
     if (!batch.empty()) {
         if (model >= 0) { std::fprintf(stderr, "-b excludes the model flags\n"); return 2; }
         if (k_set || v_set || other_set || date) {

@@ -107,6 +107,26 @@ bool read_batch_csv(const std::string &path, std::vector<BatchRow> *rows, std::s
 // the number of rows run.
 int run_batch_rows(ParamsCarrier *pc, const std::vector<BatchRow> &rows);
 
+// The settings file's values (Settings.toml schema, Input/src/TomlSettings.cpp:
+// 9-52); defaults are the shipped file's.
+struct Settings {
+    double K0 = 5e22, V = 400.0, dt = 50.0, r_injection = 1.0, injection_max = 101.0;
+    double theta_injection = 90.0;
+    bool use_uniform_injection = false;
+    double solarprop_ratio = 0.02, geliosphere_ratio = 0.2, K0_ratio = 1.0, C_delta = 8.7e-5;
+    double default_tilt_angle = 0.1;
+    bool remove_log_files = true;
+};
+// Reads a settings file (the TOML subset the schema uses); false, with the
+// reason in *err, when it cannot be read or a key is missing.
+bool read_settings(const std::string &path, Settings *s, std::string *err);
+// What a batch row uses when it names no readable settings file
+// (Algorithm/src/BatchRun.cpp:218-252: the shipped values, but
+// use_uniform_injection = true).
+Settings batch_default_settings();
+// Puts the values into the carrier as TomlSettings::parseFromSettings does.
+void put_settings(ParamsCarrier *pc, const Settings &s);
+
 // Measured values for a month and year (Input/src/MeasureValuesTransformation.cpp,
 // InputValidation.cpp:22-37,130-151): the table file of `model`, the row key
 // (1D: "<year>.<month code>", 2D: the Carrington rotation of the 15th of the

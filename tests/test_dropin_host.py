@@ -74,19 +74,77 @@ def test_cli_maps_1d_backward_like_the_reference():
     assert p["weights"][:2] == [1, 0] and p["n_weights"] == 2 and p["n_bins"] == 30
 
 
-def test_cli_1d_k0_only_when_entered_by_user():
+SETTINGS = """# the reference's schema (Input/src/TomlSettings.cpp), its shipped values
+[default_values]
+K0 = 5e22
+V = 400.0
+dt = 50.0
+r_injection = 1.0
+uniform_energy_injection_maximum = 101.0
+
+[2d_models_common_settings]
+theta_injection = 90.0
+use_uniform_injection = false
+
+[SolarProp_like_model_settings]
+SolarProp_ratio = 0.02
+
+[Geliosphere_model_settings]
+Geliosphere_ratio = 0.2
+K0_ratio = 1.0
+C_delta = 8.7e-5
+default_tilt_angle = 0.1
+
+[advanced_settings]
+remove_log_files_after_simulation = true
+"""
+
+
+def settings_file(tmp_path, **changes):
+    text = SETTINGS
+    for k, v in changes.items():
+        lines = [("%s = %s" % (k, v)) if l.split("=")[0].strip() == k else l for l in text.splitlines()]
+        text = "\n".join(lines) + "\n"
+    f = tmp_path / "settings.toml"
+    f.write_text(text)
+    return str(f)
+
+
+def test_cli_1d_k0_only_when_entered_by_user(tmp_path):
     # Without -K the 1D GPU models keep their built-in K0 (CosmicConstants.cu:16,69-73)
     assert params("-B", "--seed", "1")["K0"] == float(f32(0.000222))
     assert params("-F", "--seed", "1")["K0"] == float(f32(0.000222))
-    # defaults of Settings.toml: dt 50 s, V 400 km/s
+    # no settings file: the device defaults, dt 5 s and V 2.66667e-6 AU/s (CosmicConstants.cu:15,74-83)
     p = params("-F", "--seed", "1")
+    assert p["dt"] == 5.0 and p["V"] == float(f32(2.66667e-6))
+    # the shipped settings: dt 50 s, V 400 km/s
+    p = params("-F", "--seed", "1", "-s", settings_file(tmp_path))
     assert p["dt"] == 50.0 and p["V"] == conv_v(400)
 
 
-def test_cli_2d_models():
-    g = params("-T", "--seed", "3", "--theta", "45", "--tilt", "10", "--polarity", "-1")
+def test_settings_file_values_and_errors(tmp_path):
+    f = settings_file(tmp_path, dt="500.0", theta_injection="45.0", use_uniform_injection="true",
+                      Geliosphere_ratio="0.25", K0_ratio="2.0", C_delta="1e-4", default_tilt_angle="20.0")
+    g = params("-T", "--seed", "1", "-s", f)
+    assert g["dt"] == 500.0 and g["theta_inj"] == deg_f32(45.0)
+    assert g["injection"] == 1                                  # uniform grid injection
+    assert g["ratio"] == float(f32(0.25)) and g["delta0"] == float(f32(1e-4)) and g["tilt"] == deg_f32(20.0)
+    assert g["K0"] == float(f32(f32(conv_k0(5e22)) * f32(2.0)))
+    # a missing key is an error; a missing file only a warning
+    bad = tmp_path / "bad.toml"
+    bad.write_text(SETTINGS.replace("dt = 50.0\n", ""))
+    r = cli("-B", "-s", str(bad), "--print-params", check=False)
+    assert r.returncode != 0 and "default_values.dt" in r.stderr
+    r = cli("-B", "-s", str(tmp_path / "none.toml"), "--print-params")
+    assert "No settings file exists on entered path" in r.stderr
+
+
+def test_cli_2d_models(tmp_path):
+    # no settings: K0 stays the device default (no K0_default to scale)
+    assert params("-T", "--seed", "3")["K0"] == float(f32(0.000222))
+    g = params("-T", "--seed", "3", "--theta", "45", "--tilt", "10", "--polarity", "-1", "-s", settings_file(tmp_path))
     assert g["model"] == 3 and g["injection"] == 2          # SolarProp energies
-    assert g["K0"] == float(f32(f32(conv_k0(5e22)) * f32(1.0)))   # table K0 x K0_ratio
+    assert g["K0"] == float(f32(f32(conv_k0(5e22)) * f32(1.0)))   # settings K0 x K0_ratio
     assert g["theta_inj"] == deg_f32(45) and g["tilt"] == deg_f32(10) and g["polarity"] == -1.0
     assert g["ratio"] == float(f32(0.2)) and g["delta0"] == float(f32(8.7e-5))
     assert g["weights"][:3] == [2, 3, 4]
@@ -246,7 +304,8 @@ def test_month_year_1d_takes_v_from_the_table(tmp_path):
 def test_month_year_2d_takes_k0_tilt_polarity(tmp_path):
     app = app_with_tables(tmp_path)
     g = json.loads(run_app(app, "-T", "-m", "6", "-y", "1976", "--seed", "1", "--print-params").stdout)
-    assert g["K0"] == float(f32(f32(5.47252112676056e-05) * f32(1.0)))   # rotation 1643, x K0_ratio
+    # rotation 1643, x K0_ratio (5 when no settings file sets it, CosmicConstants.cu:108)
+    assert g["K0"] == float(f32(f32(5.47252112676056e-05) * f32(5.0)))
     assert g["tilt"] == deg_f32(6.2) and g["polarity"] == 1.0
     e = json.loads(run_app(app, "-E", "-m", "2", "-y", "2003", "--seed", "1", "--print-params").stdout)
     assert e["K0"] == float(f32(6.94405829596413e-05))                   # SolarProp table, not scaled
