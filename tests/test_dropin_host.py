@@ -323,3 +323,15 @@ def test_month_year_missing_row_and_option_rules(tmp_path):
     assert run_app(app, "-B", "-m", "6", "--print-params", check=False).returncode != 0          # -m needs -y
     assert run_app(app, "-B", "-K", "5e22", "-m", "6", "-y", "1990", "--print-params", check=False).returncode != 0
     assert run_app(app, "-B", "-V", "400", "-y", "1990", "--print-params", check=False).returncode != 0
+
+
+def test_ulysses_fixture_and_settings():
+    """The Ulysses comparison's inputs: 67 rotations of 1994-1998 with
+    Ulysses' range and colatitude, the rotation's table row and the KET
+    fluxes; the settings written for each run parse with the CLI's reader."""
+    from geliosphere_amd import ulysses
+    header, rows = ulysses.load_rotations()
+    assert header.startswith("carrington_rotation,") and len(rows) == 67
+    assert all(1.0 < r["r"] < 6.0 and 0.0 < r["theta"] < 180.0 for r in rows)
+    assert all(r["ket_p190"] > 0 and r["ket_p116"] > 0 for r in rows)
+    assert [r["cr"] for r in rows] == list(range(rows[0]["cr"], rows[0]["cr"] + 67))
