@@ -136,7 +136,7 @@ DevParams derive(const gs_params &p) {
     d.grid_step = p.grid_n ? (p.T_max - p.T_min) / p.grid_n : 0.0;
     d.sqdt = std::sqrt(p.dt);
     d.two_dt = 2.0 * p.dt;
-    d.k2dt = d.two_dt * p.K0;
+    d.k2dt = d.two_dt * p.K0 * gs::kT0;   // the 1D models' momenta are in units of T0
     const double vs = p.model == GS_MODEL_1D_BP ? -p.V : p.V;
     d.vdt = vs * p.dt;
     d.cp = (2.0 / 3.0) * vs * p.dt;

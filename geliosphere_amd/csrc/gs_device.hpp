@@ -25,12 +25,9 @@
 
 namespace gs {
 
-// Physical constants of the reference (CpuImplementations/include/Constants.hpp:63-127).
-constexpr double kM0 = 1.67261e-27;
-constexpr double kQ = 1.60219e-19;
-constexpr double kC = 2.99793e8;
+// Physical constants of the reference (CpuImplementations/include/Constants.hpp:63-127);
+// kM0, kQ, kC and kT0 are in gs_kernels.h.
 constexpr double kPi = 3.1415926535897932384626433832795;
-constexpr double kT0 = kM0 * kC * kC / (kQ * 1e9);  // proton rest energy [GeV]
 constexpr double kT0w = kM0 * kC * kC;              // [J]
 constexpr double kOmega = 2.866e-6;
 constexpr double kA = 3.4;
@@ -278,21 +275,28 @@ __device__ void fold_one(const DevParams &P, uint64_t *lds, const DevEvent &e) {
 // an event stays live in registers on the common path.  `trig` is the
 // workgroup's LDS copy of the sin/cos table (the 2D models' sin and cos).
 
-// 1D B-p.  State: r [AU], P = pc/q [GV], beta (of the last computed T, which
-// a reflection does not restore, as in OneDimensionBpCpuModel.cpp:108-140).
-// Pm.vdt = -V dt and Pm.cp = -(2/3) V dt carry the backward sign of V
-// (Constants.hpp:213-218).  T = sqrt(T0^2 + P^2) - T0 is formed only for an
-// exit; the test T < 200 is E^2 < (200 + T0)^2.
+// The 1D models carry momenta in units of T0: P = pc / (q T0 1e9), so that
+// E^2 = P^2 + 1 takes the inline constant 1.0, and Pm.k2dt = 2 K0 dt T0
+// restores K = K0 beta P [GV].  beta = P / E is unit-free.
+constexpr double kInvT0 = 1.0 / kT0;
+
+// 1D B-p.  State: r [AU], P (units of T0), beta (of the last computed T,
+// which a reflection does not restore, as in OneDimensionBpCpuModel.cpp:
+// 108-140).  Pm.vdt = -V dt and Pm.cp = -(2/3) V dt carry the backward sign
+// of V (Constants.hpp:213-218).  T = T0 (E - 1) is formed only for an exit;
+// the test T < 200 is E^2 < ((200 + T0) / T0)^2.
 struct Bp {
     static constexpr int kNormals = 1;
-    static constexpr double kE2Exit = (200.0 + kT0) * (200.0 + kT0);
+    static constexpr bool kLatencyBound = false;
+    static constexpr double kE2Exit = ((200.0 + kT0) * kInvT0) * ((200.0 + kT0) * kInvT0);
     double r, P, beta, t_inj;
     __device__ __forceinline__ void save(double *s) const { s[0] = r; s[1] = P; s[2] = beta; s[3] = t_inj; }
     __device__ __forceinline__ void load(const double *s) { r = s[0]; P = s[1]; beta = s[2]; t_inj = s[3]; }
     __device__ __forceinline__ bool init(const StepParams &Pm, double T) {
         t_inj = T;
-        P = fm::sqrt_nn(T * (T + 2.0 * kT0));
-        beta = P * fm::rcp(T + kT0);
+        const double t = T * kInvT0;
+        P = fm::sqrt_nn(t * (t + 2.0));
+        beta = P * fm::rcp(t + 1.0);
         r = Pm.r_inj;
         return r < 100.0002;
     }
@@ -303,38 +307,46 @@ struct Bp {
         const double Pn = fma(-(Pm.cp * P), inv_r, P);               // p - 2 V p dt / (3 r)
         const double dr = fma(K2dt, inv_r, z[0] * fm::sqrt_nn(K2dt)) + Pm.vdt;
         const double rn = r + dr;
-        const double rb = rn - dr;
-        const double E2 = fma(Pn, Pn, kT0 * kT0);
+        const double E2 = fma(Pn, Pn, 1.0);
         const double invE = fm::rsqrt(E2);
         const double bn = Pn * invE;
         // Goes on while beta >= 0.01 (false for a NaN state, which ends it as
         // r < 100.0002 does in the reference loop).
-        bool alive = bn >= kBetaMinBp;
+        const bool alive = bn >= kBetaMinBp;
         beta = bn;
         if (rn > 100.0) {
             // at the outer boundary (no reflection possible): exit or end
-            const bool out = bn > kBetaMinBp && E2 < kE2Exit && rb < 100.0;
-            if (out) emit(t_inj, fma(E2, invE, -kT0), rn, 0.0, 0.0);
+            const bool out = bn > kBetaMinBp && E2 < kE2Exit && rn - dr < 100.0;
+            if (out) emit(t_inj, kT0 * fma(E2, invE, -1.0), rn, 0.0, 0.0);
             r = rn;
             P = Pn;
             return alive && !out && rn < 100.0002;
         }
-        // r stays below 100.0002: rn <= 100, and a reflection restores the
-        // previous r; a reflected step keeps P
-        const bool refl = rn < kReflectBp;
-        r = refl ? rb : rn;
-        P = refl ? P : Pn;
+        // r stays below 100.0002: rn <= 100.  A reflection restores the
+        // previous r as r - dr and keeps P; it is rare, and a branch that a
+        // wave without one skips costs no selects.
+        const double P0 = P;
+        r = rn;
+        P = Pn;
+        if (rn < kReflectBp) {
+            asm volatile("");   // emits nothing; keeps the branch from becoming selects
+            r = rn - dr;
+            P = P0;
+        }
         return alive;
     }
 };
 
-// 1D F-p.  State: r, P [GV], beta and R (rigidity) of the last computed T,
-// sumac.  OneDimensionFpCpuModel.cpp:84-152.  Pm.vdt = V dt,
-// Pm.cp = (2/3) V dt, Pm.csum = (4/3) V dt.  T is formed only for a crossing;
-// the test T < 100 is E^2 < (100 + T0)^2.
+// 1D F-p.  State: r, P (units of T0), beta and R (rigidity, units of T0) of
+// the last computed T, sumac.  OneDimensionFpCpuModel.cpp:84-152.
+// Pm.vdt = V dt, Pm.cp = (2/3) V dt, Pm.csum = (4/3) V dt.  T is formed
+// only for a crossing; the test T < 100 is E^2 < ((100 + T0) / T0)^2.
 struct Fp {
     static constexpr int kNormals = 1;
-    static constexpr double kE2Cross = (100.0 + kT0) * (100.0 + kT0);
+    // A run lasts as long as its longest trajectory (a lone wave at the end),
+    // so per-step latency, not issue slots, sets its time.
+    static constexpr bool kLatencyBound = true;
+    static constexpr double kE2Cross = ((100.0 + kT0) * kInvT0) * ((100.0 + kT0) * kInvT0);
     double r, P, beta, R, sumac, t_inj;
     __device__ __forceinline__ void save(double *s) const {
         s[0] = r; s[1] = P; s[2] = beta; s[3] = R; s[4] = sumac; s[5] = t_inj;
@@ -344,9 +356,10 @@ struct Fp {
     }
     __device__ __forceinline__ bool init(const StepParams &Pm, double T) {
         t_inj = T;
-        P = fm::sqrt_nn(T * (T + 2.0 * kT0));
+        const double t = T * kInvT0;
+        P = fm::sqrt_nn(t * (t + 2.0));
         R = P;
-        beta = P * fm::rcp(T + kT0);
+        beta = P * fm::rcp(t + 1.0);
         sumac = 0.0;
         r = 100.0001;
         return true;
@@ -358,8 +371,8 @@ struct Fp {
         const double dr = fma(K2dt, inv_r, z[0] * fm::sqrt_nn(K2dt)) + Pm.vdt;
         const double Pn = fma(-(Pm.cp * P), inv_r, P);
         sumac = fma(Pm.csum, inv_r, sumac);
-        double rn = r + dr;
-        const double E2 = fma(Pn, Pn, kT0 * kT0);
+        const double rn = r + dr;
+        const double E2 = fma(Pn, Pn, 1.0);
         const double invE = fm::rsqrt(E2);
         const double rb = rn - dr;
         if (fmin(rn, rb) < 1.0) {
@@ -367,15 +380,18 @@ struct Fp {
             // (rn - 1) / ((rn - dr) - 1) < 0, without the division
             const double a = rn - 1.0, b = rb - 1.0;
             if (beta > 0.01 && E2 < kE2Cross && ((a < 0.0 && b >= 0.0) || (a > 0.0 && b < 0.0)))
-                emit(t_inj, fma(E2, invE, -kT0), rn, 0.0, sumac);
+                emit(t_inj, kT0 * fma(E2, invE, -1.0), rn, 0.0, sumac);
         }
         const bool stop = beta < 0.01;
         beta = Pn * invE;
         R = Pn;
+        // a reflected step (rn < 0.1) keeps r and P; rn < 100.0002 holds
+        // then as well, so the loop test reads the unreflected rn.  Selects,
+        // not a branch: a run's time is its longest trajectory, where a
+        // branch's latency counts and the selects' issue slots do not.
         const bool refl = rn < 0.1;
-        rn = refl ? r : rn;           // a reflected step keeps r and P
+        r = refl ? r : rn;
         P = refl ? P : Pn;
-        r = rn;
         return !stop && rn < 100.0002;
     }
 };
@@ -385,6 +401,7 @@ struct Fp {
 // 26, 32-37, 42-44).
 struct Sp {
     static constexpr int kNormals = 2;
+    static constexpr bool kLatencyBound = false;
     double r, theta, T, t_inj;
     __device__ __forceinline__ void save(double *s) const { s[0] = r; s[1] = theta; s[2] = T; s[3] = t_inj; }
     __device__ __forceinline__ void load(const double *s) { r = s[0]; theta = s[1]; T = s[2]; t_inj = s[3]; }
@@ -463,6 +480,7 @@ struct Sp {
 // not drawn; dr takes normals 1 and 2, dtheta normal 3.
 struct Ge {
     static constexpr int kNormals = 3;
+    static constexpr bool kLatencyBound = false;
     double r, theta, T, t_inj;
     __device__ __forceinline__ void save(double *s) const { s[0] = r; s[1] = theta; s[2] = T; s[3] = t_inj; }
     __device__ __forceinline__ void load(const double *s) { r = s[0]; theta = s[1]; T = s[2]; t_inj = s[3]; }

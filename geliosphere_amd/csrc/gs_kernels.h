@@ -8,6 +8,13 @@
 #include "../../include/gs_api.h"
 
 namespace gs {
+// Proton mass, charge and the speed of light as the reference writes them
+// (CpuImplementations/include/Constants.hpp:63-87), and the rest energy T0.
+constexpr double kM0 = 1.67261e-27;
+constexpr double kQ = 1.60219e-19;
+constexpr double kC = 2.99793e8;
+constexpr double kT0 = kM0 * kC * kC / (kQ * 1e9);  // [GeV]
+
 constexpr int kWave = 64;
 constexpr int kBlockThreads = 256;
 constexpr int kCycleSteps = 4;     // steps between refills (whole Philox blocks for every model)
@@ -37,7 +44,7 @@ struct StepParams {
     double K0, V, dt, r_inj, theta_inj, tilt, polarity, ratio, delta0;
     double sqdt;          // sqrt(dt)
     double two_dt;        // 2 dt
-    double k2dt;          // 2 K0 dt
+    double k2dt;          // 2 K0 dt T0 (the 1D models carry momenta in units of T0)
     double vdt;           // signed V dt as the model uses it (1D B-p: -V dt)
     double cp;            // (2/3) V dt with the same sign (momentum / energy loss)
     double csum;          // (4/3) V dt (1D F-p loss integral)

@@ -271,6 +271,14 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
             // 1D: the four normals of the cycle come from one block; every
             // normal's position is known at compile time.
             double z[4];
+            if constexpr (!kBatch && !Model::kLatencyBound) {
+                // wave-uniform Philox key: an empty asm (it emits nothing)
+                // makes the round keys be formed with scalar adds in every
+                // cycle rather than kept in 20 scalar registers over the loop
+                // (spilled to vector-register lanes, each read a VALU
+                // instruction).  A latency-bound model keeps them.
+                asm volatile("" : "+s"(k0), "+s"(k1));
+            }
             normal_block(pid, blk, k0, k1, bm, z);
             if (__ballot(active && steps + kCycleSteps > cap) == 0) {
                 // no lane can reach its step cap in this cycle
@@ -310,6 +318,7 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
                 for (int i = 0; i < NPS; ++i) {
                     if (used == 4) {
                         double t[4];
+                        if constexpr (!kBatch) asm volatile("" : "+s"(k0), "+s"(k1));   // as above
                         normal_block(pid, nb, k0, k1, bm, t);
                         b0 = t[0]; b1 = t[1]; b2 = t[2]; b3 = t[3];
                         ++nb;
