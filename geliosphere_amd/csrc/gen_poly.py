@@ -10,9 +10,10 @@ With --tables, also writes gs_bm_tables.h: the Box-Muller tables of the
 table-driven ln and sin/cos (Tang's method: a table entry near the
 argument plus a short polynomial in the remainder).
   trig:  (sin, cos)(2 pi j / 256), j = 0..255
-  log:   per interval i of the log reduction (gs_math.hpp, log_u32):
-         invc ~ 1/centre (1 exactly for the interval around 1), and
-         -ln(invc) as a double-double (hi, lo), padded to 4 doubles
+  log:   per interval i of the log reduction (gs_math.hpp, m2log_u32):
+         invc ~ 1/centre (1 exactly for the interval around 1) and
+         -ln(invc) as a double-double (hi, lo), all three times -2 (exact),
+         padded to 4 doubles
 """
 import sys
 import mpmath as mp
@@ -86,12 +87,13 @@ def write_tables(path):
         worst_r = max(worst_r, abs(lo * invc - 1), abs(hi * invc - 1))
         lc = -mp.log(invc)
         h = mp.mpf(float(lc))
-        logt += [invc, h, mp.mpf(float(lc - h)), mp.mpf(0)]
+        # stored times -2 (exact): the kernels form -2 ln u directly
+        logt += [-2 * invc, -2 * h, -2 * mp.mpf(float(lc - h)), mp.mpf(0)]
     assert worst_r < R_LOG, worst_r
     with open(path, "w") as f:
         f.write("// gs_bm_tables.h - written by gen_poly.py --tables; do not edit.\n")
         f.write("// Box-Muller tables of gs_math.hpp (sin/cos of 2 pi j/256; ln reduction\n")
-        f.write("// entries invc, -ln(invc) hi, lo, 0).  Largest |r| of the log reduction %s.\n"
+        f.write("// entries -2 invc, 2 ln(invc) hi, lo, 0).  Largest |r| of the log reduction %s.\n"
                 % mp.nstr(worst_r, 6))
         f.write("#pragma once\n\n")
         for name, vals in (("GS_BM_TRIG_TABLE", trig), ("GS_BM_LOG_TABLE", logt)):

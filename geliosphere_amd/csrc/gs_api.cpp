@@ -920,8 +920,8 @@ gs_status gs_debug_normals(gs_ctx *ctx, uint64_t seed, uint64_t id, uint64_t n, 
 }
 
 // The device kernels' Box-Muller tables, for the host evaluation.
-static const double kBmTrigHost[gs::fm::kBmTrigWords] = GS_BM_TRIG_TABLE;
-static const double kBmLogHost[gs::fm::kBmLogWords] = GS_BM_LOG_TABLE;
+alignas(16) static const double kBmTrigHost[gs::fm::kBmTrigWords] = GS_BM_TRIG_TABLE;
+alignas(16) static const double kBmLogHost[gs::fm::kBmLogWords] = GS_BM_LOG_TABLE;
 
 gs_status gs_host_math(int32_t fn, const double *x, double *y, uint64_t n) {
     if (!x || !y) return GS_ERR_INVALID_ARG;

@@ -64,9 +64,9 @@ __device__ __forceinline__ void philox10(uint32_t &x0, uint32_t &x1, uint32_t &x
 
 // Box-Muller tables (gs_bm_tables.h); every workgroup copies them into LDS
 // (load_bm_tables, then a barrier) and reads them from there.
-__constant__ double kBmTrigTable[fm::kBmTrigWords] = GS_BM_TRIG_TABLE;
-__constant__ double kBmLogTable[fm::kBmLogWords] = GS_BM_LOG_TABLE;
-struct BmTables {
+alignas(16) __constant__ double kBmTrigTable[fm::kBmTrigWords] = GS_BM_TRIG_TABLE;
+alignas(16) __constant__ double kBmLogTable[fm::kBmLogWords] = GS_BM_LOG_TABLE;
+struct alignas(16) BmTables {   // entries are read as 16-byte pairs
     double trig[fm::kBmTrigWords];
     double log[fm::kBmLogWords];
 };
@@ -84,11 +84,11 @@ __device__ __forceinline__ void normal_block(uint64_t pid, uint32_t k, uint32_t 
     uint32_t x0 = (uint32_t)pid, x1 = (uint32_t)(pid >> 32), x2 = k, x3 = 0u;
     philox10(x0, x1, x2, x3, k0, k1);
     double s, c;
-    const double ra = fm::sqrt_nn(-2.0 * fm::log_u32d(x0, bm.log));   // ln u1 <= 0
+    const double ra = fm::sqrt_nn(fm::m2log_u32d(x0, bm.log));   // -2 ln u1 >= 0
     fm::sincos_turnd(x1, bm.trig, s, c);
     z[0] = ra * c;
     z[1] = ra * s;
-    const double rb = fm::sqrt_nn(-2.0 * fm::log_u32d(x2, bm.log));
+    const double rb = fm::sqrt_nn(fm::m2log_u32d(x2, bm.log));
     fm::sincos_turnd(x3, bm.trig, s, c);
     z[2] = rb * c;
     z[3] = rb * s;

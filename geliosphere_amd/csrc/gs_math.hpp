@@ -9,7 +9,7 @@
 //   exp               k ln2 + r reduction (two-part ln2), Taylor to r^13;
 //   atan              three-point argument reduction + odd polynomial;
 //   sincos_turn       sin/cos(2 pi x 2^-32) for a 32-bit integer x and
-//   log_u32           ln((x + 1) 2^-32): the Box-Muller angle and radius
+//   m2log_u32         -2 ln((x + 1) 2^-32): the Box-Muller angle and radius
 //                     argument, and
 //   sincos_tab        sin/cos of |x| < 2^20 (Cody-Waite pi/128 reduction),
 //                     table-driven (P. T. P. Tang, "Table-driven
@@ -251,8 +251,18 @@ __host__ __device__ inline void sheet_drift(double sy, double k, double &f, doub
 }
 
 // ------------------------------------------------------------ Box-Muller
+// A 16-byte-aligned pair of table doubles, read as one 128-bit load.
+struct alignas(16) Pair {
+    double a, b;
+};
+__host__ __device__ inline Pair load_pair(const double *p) {
+    Pair v;
+    __builtin_memcpy(&v, __builtin_assume_aligned(p, 16), sizeof v);
+    return v;
+}
+
 constexpr int kBmTrigWords = 512;   // (sin, cos)(2 pi j/256), j = 0..255
-constexpr int kBmLogWords = 512;    // 128 x (invc, -ln invc hi, lo, 0)
+constexpr int kBmLogWords = 512;    // 128 x (-2 invc, 2 ln invc hi, lo, 0)
 constexpr uint64_t kLogOff = 0x3fe5f00000000000ull;   // bits of 0.685546875
 
 // sin/cos(A + a) from a table entry sA = sin A, cA = cos A and |a| <= pi/256
@@ -272,9 +282,11 @@ __host__ __device__ inline void sincos_entry(double a, double sA, double cA, dou
 // [-2^23, 2^23) exactly, a = d 2 pi 2^-32, entry j of `trig`.
 template <class Ops>
 __host__ __device__ inline void sincos_turn(uint32_t x, const double *trig, double &s, double &c) {
-    const uint32_t j = (x + 0x800000u) >> 24;
-    const int32_t d = (int32_t)(x - (j << 24));
-    sincos_entry<Ops>((double)d * kTwoPiOver2p32, trig[2 * j], trig[2 * j + 1], s, c);
+    const uint32_t xj = x + 0x800000u;
+    const uint32_t j = xj >> 24;
+    const int32_t d = (int32_t)(x - (xj & 0xff000000u));
+    const Pair t = load_pair(trig + 2 * j);
+    sincos_entry<Ops>((double)d * kTwoPiOver2p32, t.a, t.b, s, c);
 }
 
 // sin and cos of x for |x| < 2^20 from the same table: k = rint(x 128/pi),
@@ -288,39 +300,61 @@ __host__ __device__ inline void sincos_tab(double x, const double *trig, double 
     const double k = rint(x * k128OverPi);
     const double a = Ops::fma3(-k, kPi128Lo, Ops::fma3(-k, kPi128Hi, x));
     const uint32_t j = (uint32_t)(int32_t)k & 255u;
-    sincos_entry<Ops>(a, trig[2 * j], trig[2 * j + 1], s, c);
+    const Pair t = load_pair(trig + 2 * j);
+    sincos_entry<Ops>(a, t.a, t.b, s, c);
 }
 
-// ln((x + 1) 2^-32) for a 32-bit integer x, in [-22.18, 0].  v = x + 1
-// (exact) = 2^e z: tmp = bits(v) - kLogOff, e = tmp >> 52 (0 .. 32), z =
-// bits(v) - e 2^52 in [0.6855, 1.3711), interval i = bits 45..51 of tmp.
-// ln v = e ln2 + ln(1/invc_i) + ln(1 + r) with r = z invc_i - 1, |r| <= 2^-8,
-// and ln(1 + r) = r + r^2 L(r).  The interval around 1 has invc = 1 and
-// ln(1/invc) = 0, so near v = 2^32 (e = 32) the result is ln(1 + r) with
-// r = z - 1 exact: relative accuracy where the result is tiny.
+// -2 ln((x + 1) 2^-32) for a 32-bit integer x, in [0, 44.37]: the
+// Box-Muller radius squared.  v = x + 1 (exact) = 2^e z: tmp = bits(v) -
+// kLogOff, e = tmp >> 52 (0 .. 32), z = bits(v) - e 2^52 in [0.6855, 1.3711),
+// interval i = bits 45..51 of tmp.  Since kLogOff's low word is zero, only
+// the high words take part.  ln v = e ln2 + ln(1/invc_i) + ln(1 + r) with
+// r = z invc_i - 1, |r| <= 2^-8, and ln(1 + r) = r + r^2 L(r).  Every term
+// is formed times -2: the table holds -2 invc and 2 ln(invc) (hi, lo), r2 =
+// z (-2 invc) + 2 = -2 r, and -2 r^2 L(r) = r2^2 M(r2) with M's coefficients
+// those of L times (-1)^(k+1) 2^(-k-1).  Scaling by a power of two commutes
+// with rounding, so the result is -2 times the plain evaluation bit for bit,
+// without a final multiply.  The interval around 1 has invc = 1 and
+// ln(1/invc) = 0, so near v = 2^32 (e = 32) the result is -2 ln(1 + r) with
+// r2 = 2 - 2z exact: relative accuracy where the result is tiny.
+constexpr double kM2Ln2Hi = -2.0 * kLn2Hi;
+constexpr double kM2Ln2Lo = -2.0 * kLn2Lo;
+
+template <class Ops>
+__host__ __device__ inline double m2log_u32(uint32_t x, const double *logt) {
+    const uint64_t ix = double_to_bits((double)x + 1.0);   // (double)x + 1 is exact
+    const uint32_t hi = (uint32_t)(ix >> 32);
+    // the high word of tmp - 32 2^52: its exponent field is e - 32 directly,
+    // and bits 0..19 (z's mantissa bits and i) are those of tmp
+    const uint32_t thi = hi - (uint32_t)((kLogOff + (32ull << 52)) >> 32);
+    const int e32 = (int32_t)thi >> 20;
+    const uint32_t i = (thi >> 13) & 127u;
+    const uint32_t zhi = (thi & 0xfffffu) + (uint32_t)(kLogOff >> 32);
+    const double z = bits_to_double(((uint64_t)zhi << 32) | (ix & 0xffffffffull));
+    const Pair ent = load_pair(logt + 4 * i);   // -2 invc, 2 ln(invc) hi
+    const double l2_lo = logt[4 * i + 2];
+    const double r2 = Ops::fma3(z, ent.a, 2.0);
+    double p = -0.1666691666986670666717 * (-1.0 / 32.0);
+    p = Ops::fmac(p, r2, 0.2000028571784131347818 * (1.0 / 16.0));
+    p = Ops::fmac(p, r2, -0.2499999999899998399979 * (-1.0 / 8.0));
+    p = Ops::fmac(p, r2, 0.3333333333219045841247 * (1.0 / 4.0));
+    p = Ops::fmac(p, r2, -0.5 * (-1.0 / 2.0));
+    const double lp = Ops::fma3(r2 * r2, p, r2);
+    const double fe = (double)e32;
+    return Ops::fmac(fe, kM2Ln2Hi, ent.b) + (lp + Ops::fmac(fe, kM2Ln2Lo, l2_lo));
+}
+
+// ln((x + 1) 2^-32) in [-22.18, 0]: -1/2 of m2log_u32, exactly.
 template <class Ops>
 __host__ __device__ inline double log_u32(uint32_t x, const double *logt) {
-    const uint64_t ix = double_to_bits((double)x + 1.0);   // (double)x + 1 is exact
-    const uint64_t tmp = ix - kLogOff;
-    const int e = (int)((int64_t)tmp >> 52);
-    const uint32_t i = (uint32_t)(tmp >> 45) & 127u;
-    const double z = bits_to_double(ix - ((uint64_t)e << 52));
-    const double invc = logt[4 * i], lc_hi = logt[4 * i + 1], lc_lo = logt[4 * i + 2];
-    const double r = Ops::fma3(z, invc, -1.0);
-    double p = -0.1666691666986670666717;
-    p = Ops::fmac(p, r, 0.2000028571784131347818);
-    p = Ops::fmac(p, r, -0.2499999999899998399979);
-    p = Ops::fmac(p, r, 0.3333333333219045841247);
-    p = Ops::fmac(p, r, -0.5);
-    const double lp = Ops::fma3(r * r, p, r);
-    const double fe = (double)(e - 32);
-    return Ops::fmac(fe, kLn2Hi, lc_hi) + (lp + Ops::fmac(fe, kLn2Lo, lc_lo));
+    return -0.5 * m2log_u32<Ops>(x, logt);
 }
 
 // The device forms used by the kernels.
 __device__ __forceinline__ double log_fast(double x) { return log_e<DevOps>(x, 0); }
 __device__ __forceinline__ double exp_fast(double x) { return exp_e<DevOps>(x); }
 __device__ __forceinline__ double log_u32d(uint32_t x, const double *logt) { return log_u32<DevOps>(x, logt); }
+__device__ __forceinline__ double m2log_u32d(uint32_t x, const double *logt) { return m2log_u32<DevOps>(x, logt); }
 __device__ __forceinline__ void sincos_turnd(uint32_t x, const double *trig, double &s, double &c) {
     sincos_turn<DevOps>(x, trig, s, c);
 }
