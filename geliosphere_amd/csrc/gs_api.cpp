@@ -146,6 +146,14 @@ DevParams derive(const gs_params &p) {
     d.th_lo = 1.7 * kPi / 180.;
     d.th_hi = 178.3 * kPi / 180.0;
     d.pol_theta = (p.flags & GS_FLAG_SOLARPROP_GPU_DRIFT) ? 1.0 : p.polarity;
+    d.ge_k0c = p.K0 * gs::kGeC;
+    d.ge_a1 = p.ratio * gs::kGeKf / gs::kGeC;
+    d.ge_a2 = (1.0 - p.ratio) * gs::kGeKf / gs::kGeC;
+    d.ge_a3 = 3.0 * (1.0 - p.ratio) / gs::kGeC;
+    d.poldt = p.polarity * p.dt;
+    d.polthdt = d.pol_theta * p.dt;
+    d.sq2dt = std::sqrt(2.0 * p.dt);
+    d.delta0_rh = p.delta0 / gs::kRh;
     d.n_particles = p.n_particles; d.first_particle = p.first_particle; d.seed = p.seed;
     d.max_steps = p.max_steps ? p.max_steps : kDefaultMaxSteps;
     for (uint32_t i = 0; i <= p.n_bins; ++i) d.edges[i] = p.bin_edges[i];

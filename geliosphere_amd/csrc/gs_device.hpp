@@ -32,7 +32,6 @@ constexpr double kT0w = kM0 * kC * kC;              // [J]
 constexpr double kOmega = 2.866e-6;
 constexpr double kA = 3.4;
 constexpr double kKonvF = 9.0e-5 / 2.0;
-constexpr double kRh = 695510.0 / 150000000.0;
 // Float literals of the reference 1D B-p model, compared against doubles
 // (OneDimensionBpCpuModel.cpp:116,130,136).
 constexpr double kBetaMinBp = (double)0.01f;
@@ -414,6 +413,7 @@ struct Sp {
     }
     template <class Emit>
     __device__ __forceinline__ bool step(const StepParams &Pm, const double *z, const double *trig, const Emit &emit) {
+#pragma clang fp contract(fast)
         double s, c;
         fm::sincosd(theta, trig, s, c);
         const double Tp = T + 2.0 * kT0;
@@ -427,37 +427,37 @@ struct Sp {
         const double beta = Rig * inv_E;
         const double sq = fm::sqrt_nn(opg);
         const double inv_sq = sq * inv_opg;
+        const double inv_opg15 = inv_opg * inv_sq;
+        const double inv_opg25 = inv_opg * inv_opg15;
         const double Rc = (Rig < 0.1) ? 0.1 : Rig;
-        const double kb = Pm.K0 * beta;
+        const double kbc = Pm.ge_k0c * beta;   // K0 beta (5/3.4)/3
         // K_par = K0 beta max(P, 0.1) (5/3.4) r^2 / (3 sqrt(1 + gamma^2))
-        const double Kpar = kb * Rc * r2 * inv_sq * ((5.0 / 3.4) / 3.0);
+        const double Kpar = kbc * Rc * r2 * inv_sq;
         const double Kper = Pm.ratio * Kpar;
         const double Krr = fma(Kpar - Kper, inv_opg, Kper);
-        // dK_rr/dr with r^2 dKrrTmp = 2 r gamma^2 (dKrrTmp = 2 r Omega^2 sin^2 / V^2):
-        //   t1 = ratio K0 beta P r (2 sq - gamma^2 / sq) / (1 + gamma^2)
-        //   t2 = (1 - ratio) K0 beta P r sq (2 (1 + gamma^2) - 3 gamma^2) / (1 + gamma^2)^3
-        const double kr = kb * Rig * r;
-        const double t1 = Pm.ratio * kr * fma(-g2, inv_sq, 2.0 * sq) * inv_opg;
-        const double t2 = (1.0 - Pm.ratio) * kr * sq * fma(-3.0, g2, 2.0 * opg) * (inv_opg * inv_opg * inv_opg);
-        const double dKrr = (t1 + t2) * (5.0 / (3.0 * 3.4));
+        // dK_rr/dr (SolarPropLikeCpuModel.cpp:150-163) with r^2 dKrrTmp = 2 r gamma^2:
+        //   5/(3 3.4) K0 beta P r [ratio (2 sqrt(1+g^2) - g^2/sqrt(1+g^2)) / (1+g^2)
+        //                          + (1 - ratio) sqrt(1+g^2) (2 (1+g^2) - 3 g^2) / (1+g^2)^3]
+        // = C K0 beta P r [ratio (2 + g^2) / (1+g^2)^1.5 + (1 - ratio) (2 - g^2) / (1+g^2)^2.5],
+        // since 5/(3 3.4) = (5/3.4)/3 = C.
+        const double krc = kbc * Rig * r;
+        const double dKrr = krc * fma(Pm.ratio * (2.0 + g2), inv_opg15, (1.0 - Pm.ratio) * (2.0 - g2) * inv_opg25);
         const double sKrr = fm::sqrt_nn(Pm.two_dt * Krr);
         const double sKtt = fm::sqrt_nn(Pm.two_dt * Kper);
         double dr = fma(fma(2.0 * Krr, inv_r, dKrr) - Pm.V, Pm.dt, z[0] * sKrr);
-        const double csr = c * inv_s * inv_r;
-        double dth = fma(Kper * csr * inv_r * inv_opg, Pm.dt, z[1] * sKtt * inv_r);
+        const double cs = c * inv_s;
+        double dth = fma(Kper * cs * inv_r * inv_r * inv_opg, Pm.dt, z[1] * sKtt * inv_r);
         const double dTkin = -Pm.cp * (Tp * inv_E) * T * inv_r;
         // drifts: sheet angle alphaM + 2 Larmor pi / (180 r), Larmor = (0.0225 / A) P r^2 / sq
         const double sy = fm::sind(fma(Rig * r * inv_sq, (0.0225 / kA) * (kPi / 90.0), Pm.tilt), trig);
         double f, g;
         fm::sheet_drift<fm::DevOps>(sy, 1.0 - theta * (2.0 / kPi), f, g);
         const double rbg = Rig * beta * r * gamma * inv_opg;
-        const double dc = (kKonvF * (2.0 / (3.0 * kA))) * rbg * f * inv_opg;
-        const double DriftR = Pm.polarity * dc * c * inv_s;
-        const double DriftTheta = -Pm.pol_theta * dc * (2.0 + g2);
-        // fprime = -(2/pi) g
-        const double DriftSheetR = Pm.polarity * (kKonvF * (1.0 / (3.0 * kA)) * (-2.0 / kPi)) * rbg * g;
-        dr = fma(DriftR + DriftSheetR, Pm.dt, dr);
-        dth = fma(DriftTheta * Pm.dt, inv_r, dth);
+        const double dc = rbg * ((kKonvF * (2.0 / (3.0 * kA))) * f) * inv_opg;
+        // (DriftR + DriftSheetR) dt, with fprime = -(2/pi) g
+        dr = fma(Pm.poldt, fma(dc, cs, (kKonvF * (1.0 / (3.0 * kA)) * (-2.0 / kPi)) * rbg * g), dr);
+        // DriftTheta dt / r
+        dth = fma(-Pm.polthdt * dc * (2.0 + g2), inv_r, dth);
         r = r + dr;
         theta = theta + dth;
         if (theta < 0.0) theta = fabs(theta);
@@ -493,7 +493,15 @@ struct Ge {
     }
     template <class Emit>
     __device__ __forceinline__ bool step(const StepParams &Pm, const double *z, const double *trig, const Emit &emit) {
-        constexpr double kf = 1.0 / (3.0 * (5.0 / 3.4));   // as written in GeliosphereCpuModel.cpp:210-299
+#pragma clang fp contract(fast)
+        // The reference's factors are regrouped so that each per-simulation
+        // product is formed once on the host (StepParams ge_*): with
+        // C = (5/3.4)/3, kf = 1/(3 (5/3.4)) and kbrc = C K0 beta P,
+        //   dK_per term  ratio kf K0 beta P r (2 + h) / Cb^1.5     (2 sqrt(Cb) - h/sqrt(Cb) = (2 + h)/sqrt(Cb))
+        //   dK_rr term   (1 - ratio) kf K0 beta P r (2 - h) / Cb^2.5   (2 Cb - 3 h = 2 - h)
+        //   s c r^2 omega^2 / V^2 = gamma^2 cos/sin
+        // with h = Cb - 1; the noise coefficients carry sqrt(dt) and the
+        // drifts polarity dt.
         double s, c;
         fm::sincosd(theta, trig, s, c);
         const double Tp = T + 2.0 * kT0;
@@ -512,12 +520,12 @@ struct Ge {
         const double beta = Rig * inv_E;
         const bool polar = (theta < Pm.th_lo) || (theta > Pm.th_hi);
         const bool band = (theta > Pm.th_lo) && (theta < Pm.th_hi);
-        const double delta = polar ? 0.003 : Pm.delta0 * inv_s;
-        const double deltarh = delta * (1.0 / kRh);
+        const double deltarh = polar ? 0.003 * (1.0 / kRh) : Pm.delta0_rh * inv_s;   // delta / rh
         const double deltarh2 = deltarh * deltarh;
         const double gamma = r * Pm.omega_v * s;
         const double g2 = gamma * gamma;
-        const double h = fma(r2, deltarh2, g2);       // Cb - 1
+        const double rd2 = r2 * deltarh2;
+        const double h = rd2 + g2;                    // Cb - 1
         const double Cb = 1.0 + h;
         const double inv_Cb = fm::rcp(Cb);
         const double sqCb = fm::sqrt_nn(Cb);
@@ -525,55 +533,47 @@ struct Ge {
         const double inv_Cb15 = inv_Cb * inv_sqCb;
         const double inv_Cb25 = inv_Cb * inv_Cb15;
         const double Rc = (Rig < 0.1) ? 0.1 : Rig;
-        const double kb = Pm.K0 * beta;
-        const double Kpar = kb * Rc * r2 * inv_sqCb * ((5.0 / 3.4) / 3.0);
+        const double kbc = Pm.ge_k0c * beta;          // C K0 beta
+        const double Kpar = kbc * Rc * r2 * inv_sqCb;
         const double Kper = Pm.ratio * Kpar;
         const double dKc = (Kpar - Kper) * inv_Cb;    // (K_par - K_per) / Cb
         const double Krr = Kper + dKc;
-        const double Ktt = fma(r2 * deltarh2, dKc, Kper);
+        const double Ktt = fma(rd2, dKc, Kper);
         const double Krt = deltarh * r * dKc;
-        // 1/Ktt and sqrt(2 Ktt) from one rsqrt
+        // 1/Ktt and sqrt(2 Ktt dt) from one rsqrt
         const double rK = fm::rsqrt(Ktt);
         const double inv_Ktt = rK * rK;
-        const double sq2Ktt = (1.4142135623730951 * Ktt) * rK;
-        const double B11 = fm::sqrt_pos(2.0 * fma(Krr, Ktt, -(Krt * Krt)) * inv_Ktt);
-        const double B12 = (1.4142135623730951 * Krt) * rK;   // Krt sqrt(2 Ktt) / Ktt
-        const double B22 = sq2Ktt * inv_r;
-        const double kbr = kb * Rig;                          // K0 beta P
-        const double kr = kbr * r;
-        // r^2 COmega = 2 r (gamma^2 + r^2 deltarh^2) = 2 r h
-        const double dKper = Pm.ratio * kr * fma(-h, inv_sqCb, 2.0 * sqCb) * kf * inv_Cb;
-        const double dKrr = fma((1.0 - Pm.ratio) * kr * sqCb, fma(-3.0, h, 2.0 * Cb) * kf * (inv_Cb * inv_Cb * inv_Cb),
-                                dKper);
-        const double r2w = Pm.omega2_v2 * r2;
-        double dKtt;
-        if (band) {
-            const double CKtt = s * c * r2w;
-            const double dKtt1 = -Pm.ratio * kbr * r2 * CKtt * kf * inv_Cb15;
-            const double dKtt2 = (1.0 - Pm.ratio) * kbr * r2 * r2 * deltarh2;
-            const double dKtt4 = 3.0 * CKtt * inv_Cb25;
-            dKtt = dKtt1 - dKtt2 * dKtt4;
-        } else {
+        const double B11 = fm::sqrt_pos(Pm.two_dt * fma(Krr, Ktt, -(Krt * Krt)) * inv_Ktt);   // times sqrt(dt)
+        const double B12 = (Pm.sq2dt * Krt) * rK;
+        const double B22 = (Pm.sq2dt * Ktt) * rK * inv_r;
+        const double kbrc = kbc * Rig;                // C K0 beta P
+        const double krc = kbrc * r;
+        const double kr2c = kbrc * r2;
+        const double csi = c * inv_s;
+        const double dKper = (Pm.ge_a1 * krc) * (2.0 + h) * inv_Cb15;
+        const double dKrr = fma(Pm.ge_a2 * krc, (2.0 - h) * inv_Cb25, dKper);
+        const double q = Pm.ge_a2 * kr2c * inv_Cb25;
+        const double dKrtr = q * deltarh;
+        // between the polar caps (the reference's band branches)
+        double CKtt = g2 * csi;                       // s c r^2 omega^2 / V^2
+        double dKtt = -(CKtt * kr2c) * fma(Pm.ge_a3 * rd2, inv_Cb25, Pm.ge_a1 * inv_Cb15);
+        double dKrtt = -3.0 * (dKrtr * r) * CKtt;
+        if (!band) {
+            asm volatile("");   // emits nothing; keeps this rare branch from becoming selects
+            // GeliosphereCpuModel.cpp:244-279,295-300 (delta = delta0 / sin)
+            const double d2 = Pm.delta0_rh * Pm.delta0_rh;
             const double inv_s2 = inv_s * inv_s;
             const double inv_s3 = inv_s2 * inv_s;
-            const double CKtt = s * c * r2w - r2 * (Pm.delta0 * Pm.delta0 / (kRh * kRh)) * c * inv_s3;
-            const double dKtt1 = -Pm.ratio * kbr * r2 * CKtt * kf * inv_Cb15;
-            const double dKtt2 = (1.0 - Pm.ratio) * kbr * r2 * r2 * (Pm.delta0 * Pm.delta0 / (kRh * kRh));
-            const double dKtt3 = -2.0 * c * inv_s3 * kf * inv_Cb15;
-            const double dKtt4 = 3.0 * CKtt * inv_s2 * kf * inv_Cb25;
+            CKtt = CKtt - r2 * d2 * c * inv_s3;
+            const double dKtt1 = -Pm.ge_a1 * kr2c * CKtt * inv_Cb15;
+            const double dKtt2 = (Pm.ge_a3 * (1.0 / 3.0)) * kr2c * r2 * d2;
+            const double dKtt3 = -2.0 * c * inv_s3 * kGeKf * inv_Cb15;
+            const double dKtt4 = 3.0 * CKtt * inv_s2 * kGeKf * inv_Cb25;
             dKtt = dKtt1 + dKtt2 * (dKtt3 - dKtt4);
+            dKrtt = -q * r * Pm.delta0_rh * c * inv_s2 * (1.0 - (2.0 * rd2) + (4.0 * g2));
         }
-        const double q = (1.0 - Pm.ratio) * kbr * r2 * kf * inv_Cb25;
-        const double dKrtr = q * deltarh;
-        const double krtt0 = q * r * (1.0 / kRh);
-        const double dKrtt = band ? -krtt0 * delta * 3.0 * g2 * c * inv_s
-                                  : -krtt0 * Pm.delta0 * c * inv_s * inv_s *
-                                        (1.0 - (2.0 * r2 * deltarh2) + (4.0 * g2));
-        const double csi = c * inv_s;
-        double dr = fma(fma(fma(Krt, csi, 2.0 * Krr + dKrtt), inv_r, dKrr) - Pm.V, Pm.dt,
-                        fma(z[0], B11, z[1] * B12) * Pm.sqdt);
-        double dth = fma(fma(fma(Ktt, csi, dKtt), inv_r, 2.0 * Krt) * inv_r + dKrtr, Pm.dt,
-                         z[2] * B22 * Pm.sqdt);
+        double dr = fma(fma(fma(Krt, csi, 2.0 * Krr + dKrtt), inv_r, dKrr) - Pm.V, Pm.dt, fma(z[0], B11, z[1] * B12));
+        double dth = fma(fma(fma(Ktt, csi, dKtt), inv_r, 2.0 * Krt) * inv_r + dKrtr, Pm.dt, z[2] * B22);
         const double dTkin = -Pm.cp * (Tp * inv_E) * T * inv_r;
         // drifts (Kappl 2016 Eq. 26, 32-37 with Cb in place of 1 + gamma^2); sheet
         // angle alphaM + 2 Larmor pi / (180 r), Larmor = (0.0225 / A) P r^2 / sqrt(Cb)
@@ -581,13 +581,11 @@ struct Ge {
         double f, g;
         fm::sheet_drift<fm::DevOps>(sy, 1.0 - theta * (2.0 / kPi), f, g);
         const double rbg = Rig * beta * r * gamma * inv_Cb;
-        const double dc = (kKonvF * (2.0 / (3.0 * kA))) * rbg * f * inv_Cb;
-        const double DriftR = Pm.polarity * dc * csi;
-        const double DriftTheta = -Pm.polarity * dc * (2.0 + g2);
-        // fprime = -(2/pi) g
-        const double DriftSheetR = Pm.polarity * (kKonvF * (1.0 / (3.0 * kA)) * (-2.0 / kPi)) * rbg * g;
-        dr = fma(DriftR + DriftSheetR, Pm.dt, dr);
-        dth = fma(DriftTheta * Pm.dt, inv_r, dth);
+        const double dc = rbg * ((kKonvF * (2.0 / (3.0 * kA))) * f) * inv_Cb;
+        // (DriftR + DriftSheetR) dt, with fprime = -(2/pi) g
+        dr = fma(Pm.poldt, fma(dc, csi, (kKonvF * (1.0 / (3.0 * kA)) * (-2.0 / kPi)) * rbg * g), dr);
+        // DriftTheta dt / r
+        dth = fma(-Pm.poldt * dc * (2.0 + g2), inv_r, dth);
         r = r + dr;
         theta = theta + dth;
         T = T - dTkin;

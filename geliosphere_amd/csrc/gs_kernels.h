@@ -14,6 +14,9 @@ constexpr double kM0 = 1.67261e-27;
 constexpr double kQ = 1.60219e-19;
 constexpr double kC = 2.99793e8;
 constexpr double kT0 = kM0 * kC * kC / (kQ * 1e9);  // [GeV]
+constexpr double kRh = 695510.0 / 150000000.0;      // solar radius [AU] (Constants.hpp:127)
+constexpr double kGeC = (5.0 / 3.4) / 3.0;           // K_par factor (GeliosphereCpuModel.cpp:167)
+constexpr double kGeKf = 1.0 / (3.0 * (5.0 / 3.4));  // derivative factor (GeliosphereCpuModel.cpp:210-299)
 
 constexpr int kWave = 64;
 constexpr int kBlockThreads = 256;
@@ -52,6 +55,18 @@ struct StepParams {
     double omega2_v2;     // omega^2 / V^2
     double th_lo, th_hi;  // 1.7 deg and 178.3 deg in rad, as the reference writes them
     double pol_theta;     // polarity used in the 2D theta drift
+    // 2D products of per-simulation and fixed factors (the step multiplies
+    // each once; with C = (5/3.4)/3 and kf = 1/(3 (5/3.4)), the factors
+    // GeliosphereCpuModel.cpp:167-299 and SolarPropLikeCpuModel.cpp:131-163
+    // write around K0 and K0 beta P):
+    double ge_k0c;        // K0 C (both 2D models)
+    double ge_a1;         // ratio kf / C
+    double ge_a2;         // (1 - ratio) kf / C
+    double ge_a3;         // 3 (1 - ratio) / C
+    double poldt;         // polarity dt (2D drifts)
+    double polthdt;       // pol_theta dt (2D SolarProp theta drift)
+    double sq2dt;         // sqrt(2 dt)
+    double delta0_rh;     // delta0 / rh
 };
 
 // Scheduling order of a run's particles (gs_launch_order, batches): offsets
