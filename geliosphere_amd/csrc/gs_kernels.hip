@@ -303,8 +303,29 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
                     }
                 }
             }
+        } else if constexpr (NPS == 2) {
+            // 2D SolarProp: one block serves two steps (normals 4k .. 4k+3 of
+            // the cycle); the loop over step pairs stays rolled.
+#pragma unroll 1
+            for (int s = 0; s < kCycleSteps; s += 2) {
+                double t[4];
+                if constexpr (!kBatch) asm volatile("" : "+s"(k0), "+s"(k1));   // as above
+                normal_block(pid, blk + (uint32_t)(s >> 1), k0, k1, bm, t);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    if (active) {
+                        if (steps >= cap) {
+                            ++my_cap;
+                            active = false;
+                        } else {
+                            ++steps;
+                            if (!do_step(&t[2 * h])) active = false;
+                        }
+                    }
+                }
+            }
         } else {
-            // 2D: steps stay rolled (the step body is large); normals come from
+            // 2D Geliosphere: steps stay rolled (the step body is large); normals come from
             // a four-entry buffer refilled from the next block when empty.  All
             // lanes are block-aligned at the cycle start, so the refill branch
             // is wave-uniform.
