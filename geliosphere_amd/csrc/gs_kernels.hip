@@ -681,7 +681,10 @@ using namespace gs;
 #ifndef GS_PLAIN_WAVES_SP
 #define GS_PLAIN_WAVES_SP 1
 #endif
-constexpr int kBatchWaves1D = 1, kBatchWavesSp = 2, kBatchWavesGe = GS_BATCH_WAVES_GE;
+#ifndef GS_BATCH_WAVES_BP
+#define GS_BATCH_WAVES_BP 5   // 96 VGPRs, no spills
+#endif
+constexpr int kBatchWaves1D = 1, kBatchWavesBp = GS_BATCH_WAVES_BP, kBatchWavesSp = 2, kBatchWavesGe = GS_BATCH_WAVES_GE;
 constexpr int kPlainWavesBp = GS_PLAIN_WAVES_BP;
 constexpr int kPlainWaves1D = GS_PLAIN_WAVES_1D, kPlainWavesSp = GS_PLAIN_WAVES_SP, kPlainWavesGe = GS_PLAIN_WAVES_GE;
 
@@ -690,7 +693,7 @@ static const void *traj_kernel_ptr(int model, bool batch) {
     case GS_MODEL_1D_FP:
         return batch ? (const void *)&batch_kernel<Fp, kBatchWaves1D> : (const void *)&plain_kernel<Fp, kPlainWaves1D>;
     case GS_MODEL_1D_BP:
-        return batch ? (const void *)&batch_kernel<Bp, kBatchWaves1D> : (const void *)&plain_kernel<Bp, kPlainWavesBp>;
+        return batch ? (const void *)&batch_kernel<Bp, kBatchWavesBp> : (const void *)&plain_kernel<Bp, kPlainWavesBp>;
     case GS_MODEL_2D_SOLARPROP:
         return batch ? (const void *)&batch_kernel<Sp, kBatchWavesSp> : (const void *)&plain_kernel<Sp, kPlainWavesSp>;
     default:
@@ -735,7 +738,7 @@ hipError_t gs_launch_traj(int model, bool batch, const DevParams *sims, uint32_t
         return hipErrorInvalidValue;
     switch (model) {
     case GS_MODEL_1D_FP: launch_traj_t<Fp, kBatchWaves1D, kPlainWaves1D>(batch, sims, n_sims, offsets, O, grid, s); break;
-    case GS_MODEL_1D_BP: launch_traj_t<Bp, kBatchWaves1D, kPlainWavesBp>(batch, sims, n_sims, offsets, O, grid, s); break;
+    case GS_MODEL_1D_BP: launch_traj_t<Bp, kBatchWavesBp, kPlainWavesBp>(batch, sims, n_sims, offsets, O, grid, s); break;
     case GS_MODEL_2D_SOLARPROP: launch_traj_t<Sp, kBatchWavesSp, kPlainWavesSp>(batch, sims, n_sims, offsets, O, grid, s); break;
     case GS_MODEL_2D_GELIOSPHERE: launch_traj_t<Ge, kBatchWavesGe, kPlainWavesGe>(batch, sims, n_sims, offsets, O, grid, s); break;
     default: return hipErrorInvalidValue;
