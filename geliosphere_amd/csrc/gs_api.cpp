@@ -136,7 +136,7 @@ DevParams derive(const gs_params &p) {
     d.grid_step = p.grid_n ? (p.T_max - p.T_min) / p.grid_n : 0.0;
     d.sqdt = std::sqrt(p.dt);
     d.two_dt = 2.0 * p.dt;
-    d.k2dt = d.two_dt * p.K0 * gs::kT0;   // the 1D models' momenta are in units of T0
+    d.sk2dt = std::sqrt(d.two_dt * p.K0 * gs::kT0);   // the 1D models' momenta are in units of T0
     const double vs = p.model == GS_MODEL_1D_BP ? -p.V : p.V;
     d.vdt = vs * p.dt;
     d.cp = (2.0 / 3.0) * vs * p.dt;
@@ -895,7 +895,7 @@ gs_status gs_run_plain(gs_ctx *ctx, const gs_params *p, gs_spectrum *out, gs_exi
 
 gs_status gs_debug_math(gs_ctx *ctx, int32_t fn, const double *x, double *y, uint64_t n) {
     if (!ctx || (!x && fn != GS_DBG_NORMALS) || !y) return GS_ERR_INVALID_ARG;
-    if (fn < GS_DBG_RCP || fn > GS_DBG_SHEET_G) return fail(ctx, GS_ERR_INVALID_ARG, "unknown fn");
+    if (fn < GS_DBG_RCP || fn > GS_DBG_RSQRT4) return fail(ctx, GS_ERR_INVALID_ARG, "unknown fn");
     if (!x) return GS_ERR_INVALID_ARG;
     if (n == 0) return GS_OK;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -949,7 +949,7 @@ gs_status gs_host_math(int32_t fn, const double *x, double *y, uint64_t n) {
         case GS_DBG_ATAN_FRAC: y[i] = fm::atan_frac<H>(x[i], x[n + i]); break;
         case GS_DBG_SHEET_F: fm::sheet_drift<H>(x[i], x[n + i], s, c); y[i] = s; break;
         case GS_DBG_SHEET_G: fm::sheet_drift<H>(x[i], x[n + i], s, c); y[i] = c; break;
-        default: return GS_ERR_UNSUPPORTED;   // rcp / rsqrt / sqrt seeds exist on the device only
+        default: return GS_ERR_UNSUPPORTED;   // rcp / rsqrt / sqrt / rsqrt4 seeds exist on the device only
         }
     }
     return GS_OK;

@@ -275,115 +275,127 @@ __device__ void fold_one(const DevParams &P, uint64_t *lds, const DevEvent &e) {
 // workgroup's LDS copy of the sin/cos table (the 2D models' sin and cos).
 
 // The 1D models carry momenta in units of T0: P = pc / (q T0 1e9), so that
-// E^2 = P^2 + 1 takes the inline constant 1.0, and Pm.k2dt = 2 K0 dt T0
-// restores K = K0 beta P [GV].  beta = P / E is unit-free.
+// E^2 = P^2 + 1 takes the inline constant 1.0.  K = K0 beta P [GV] enters
+// only as sqrt(2 K dt) = sqrt(2 K0 dt T0) w with w = sqrt(beta P)
+// (Pm.sk2dt = sqrt(2 K0 dt T0)), and as 2 K dt / r = (sqrt(2 K dt))^2 / r.
+// With beta = P / E the common case has w = P E^(-1/2) = P q and beta =
+// P q^2, q = (P^2 + 1)^(-1/4) from one rsqrt4: one transcendental chain per
+// step instead of a 1/sqrt(E^2) and a sqrt(2 K dt).
 constexpr double kInvT0 = 1.0 / kT0;
 
-// 1D B-p.  State: r [AU], P (units of T0), beta (of the last computed T,
-// which a reflection does not restore, as in OneDimensionBpCpuModel.cpp:
-// 108-140).  Pm.vdt = -V dt and Pm.cp = -(2/3) V dt carry the backward sign
-// of V (Constants.hpp:213-218).  T = T0 (E - 1) is formed only for an exit;
-// the test T < 200 is E^2 < ((200 + T0) / T0)^2.
+// 1D B-p.  State: r [AU], P (units of T0), w = sqrt(beta P) with beta of
+// the last computed T, which a reflection does not restore
+// (OneDimensionBpCpuModel.cpp:108-140).  Pm.vdt = -V dt and Pm.cp =
+// -(2/3) V dt carry the backward sign of V (Constants.hpp:213-218).
+// T = T0 (E - 1) is formed only for an exit; the test T < 200 is
+// E^2 < ((200 + T0) / T0)^2.
 struct Bp {
     static constexpr int kNormals = 1;
     static constexpr bool kLatencyBound = false;
     static constexpr double kE2Exit = ((200.0 + kT0) * kInvT0) * ((200.0 + kT0) * kInvT0);
-    double r, P, beta, t_inj;
-    __device__ __forceinline__ void save(double *s) const { s[0] = r; s[1] = P; s[2] = beta; s[3] = t_inj; }
-    __device__ __forceinline__ void load(const double *s) { r = s[0]; P = s[1]; beta = s[2]; t_inj = s[3]; }
+    double r, P, w, t_inj;
+    __device__ __forceinline__ void save(double *s) const { s[0] = r; s[1] = P; s[2] = w; s[3] = t_inj; }
+    __device__ __forceinline__ void load(const double *s) { r = s[0]; P = s[1]; w = s[2]; t_inj = s[3]; }
     __device__ __forceinline__ bool init(const StepParams &Pm, double T) {
         t_inj = T;
         const double t = T * kInvT0;
         P = fm::sqrt_nn(t * (t + 2.0));
-        beta = P * fm::rcp(t + 1.0);
+        w = P * fm::rsqrt(t + 1.0);   // E = t + 1
         r = Pm.r_inj;
         return r < 100.0002;
     }
     template <class Emit>
     __device__ __forceinline__ bool step(const StepParams &Pm, const double *z, const double *trig, const Emit &emit) {
-        const double K2dt = Pm.k2dt * (beta * P);                     // 2 K dt
         const double inv_r = fm::rcp(r);
+        const double sK = Pm.sk2dt * w;                                // sqrt(2 K dt)
         const double Pn = fma(-(Pm.cp * P), inv_r, P);               // p - 2 V p dt / (3 r)
-        const double dr = fma(K2dt, inv_r, z[0] * fm::sqrt_nn(K2dt)) + Pm.vdt;
+        const double dr = sK * fma(sK, inv_r, z[0]) + Pm.vdt;       // (V + 2K/r) dt + z sqrt(2 K dt)
         const double rn = r + dr;
         const double E2 = fma(Pn, Pn, 1.0);
-        const double invE = fm::rsqrt(E2);
+        const double q = fm::rsqrt4(E2);
+        const double invE = q * q;
         const double bn = Pn * invE;
+        const double wn = Pn * q;
         // Goes on while beta >= 0.01 (false for a NaN state, which ends it as
         // r < 100.0002 does in the reference loop).
         const bool alive = bn >= kBetaMinBp;
-        beta = bn;
         if (rn > 100.0) {
             // at the outer boundary (no reflection possible): exit or end
             const bool out = bn > kBetaMinBp && E2 < kE2Exit && rn - dr < 100.0;
             if (out) emit(t_inj, kT0 * fma(E2, invE, -1.0), rn, 0.0, 0.0);
             r = rn;
             P = Pn;
+            w = wn;
             return alive && !out && rn < 100.0002;
         }
         // r stays below 100.0002: rn <= 100.  A reflection restores the
-        // previous r as r - dr and keeps P; it is rare, and a branch that a
-        // wave without one skips costs no selects.
+        // previous r as r - dr and keeps P, while beta stays that of Pn; it
+        // is rare, and a branch that a wave without one skips costs no
+        // selects.
         const double P0 = P;
         r = rn;
         P = Pn;
+        w = wn;
         if (rn < kReflectBp) {
             asm volatile("");   // emits nothing; keeps the branch from becoming selects
             r = rn - dr;
             P = P0;
+            w = fm::sqrt_nn(bn * P0);
         }
         return alive;
     }
 };
 
-// 1D F-p.  State: r, P (units of T0), beta and R (rigidity, units of T0) of
-// the last computed T, sumac.  OneDimensionFpCpuModel.cpp:84-152.
-// Pm.vdt = V dt, Pm.cp = (2/3) V dt, Pm.csum = (4/3) V dt.  T is formed
-// only for a crossing; the test T < 100 is E^2 < ((100 + T0) / T0)^2.
+// 1D F-p.  State: r, P (units of T0), beta of the last computed T, w =
+// sqrt(beta R) with R = that T's momentum (a reflection keeps P but not
+// beta or R), sumac.  OneDimensionFpCpuModel.cpp:84-152.  Pm.vdt = V dt,
+// Pm.cp = (2/3) V dt, Pm.csum = (4/3) V dt.  T is formed only for a
+// crossing; the test T < 100 is E^2 < ((100 + T0) / T0)^2.
 struct Fp {
     static constexpr int kNormals = 1;
     // A run lasts as long as its longest trajectory (a lone wave at the end),
     // so per-step latency, not issue slots, sets its time.
     static constexpr bool kLatencyBound = true;
     static constexpr double kE2Cross = ((100.0 + kT0) * kInvT0) * ((100.0 + kT0) * kInvT0);
-    double r, P, beta, R, sumac, t_inj;
+    double r, P, beta, w, sumac, t_inj;
     __device__ __forceinline__ void save(double *s) const {
-        s[0] = r; s[1] = P; s[2] = beta; s[3] = R; s[4] = sumac; s[5] = t_inj;
+        s[0] = r; s[1] = P; s[2] = beta; s[3] = w; s[4] = sumac; s[5] = t_inj;
     }
     __device__ __forceinline__ void load(const double *s) {
-        r = s[0]; P = s[1]; beta = s[2]; R = s[3]; sumac = s[4]; t_inj = s[5];
+        r = s[0]; P = s[1]; beta = s[2]; w = s[3]; sumac = s[4]; t_inj = s[5];
     }
     __device__ __forceinline__ bool init(const StepParams &Pm, double T) {
         t_inj = T;
         const double t = T * kInvT0;
         P = fm::sqrt_nn(t * (t + 2.0));
-        R = P;
-        beta = P * fm::rcp(t + 1.0);
+        const double q2 = fm::rcp(t + 1.0);   // 1/E
+        beta = P * q2;
+        w = P * fm::rsqrt(t + 1.0);
         sumac = 0.0;
         r = 100.0001;
         return true;
     }
     template <class Emit>
     __device__ __forceinline__ bool step(const StepParams &Pm, const double *z, const double *trig, const Emit &emit) {
-        const double K2dt = Pm.k2dt * (beta * R);
         const double inv_r = fm::rcp(r);
-        const double dr = fma(K2dt, inv_r, z[0] * fm::sqrt_nn(K2dt)) + Pm.vdt;
+        const double sK = Pm.sk2dt * w;
+        const double dr = sK * fma(sK, inv_r, z[0]) + Pm.vdt;
         const double Pn = fma(-(Pm.cp * P), inv_r, P);
         sumac = fma(Pm.csum, inv_r, sumac);
         const double rn = r + dr;
         const double E2 = fma(Pn, Pn, 1.0);
-        const double invE = fm::rsqrt(E2);
+        const double q = fm::rsqrt4(E2);
         const double rb = rn - dr;
         if (fmin(rn, rb) < 1.0) {
             // a crossing of 1 AU needs one of rn, r - dr below 1:
             // (rn - 1) / ((rn - dr) - 1) < 0, without the division
             const double a = rn - 1.0, b = rb - 1.0;
             if (beta > 0.01 && E2 < kE2Cross && ((a < 0.0 && b >= 0.0) || (a > 0.0 && b < 0.0)))
-                emit(t_inj, kT0 * fma(E2, invE, -1.0), rn, 0.0, sumac);
+                emit(t_inj, kT0 * fma(E2, q * q, -1.0), rn, 0.0, sumac);
         }
         const bool stop = beta < 0.01;
-        beta = Pn * invE;
-        R = Pn;
+        w = Pn * q;
+        beta = w * q;
         // a reflected step (rn < 0.1) keeps r and P; rn < 100.0002 holds
         // then as well, so the loop test reads the unreflected rn.  Selects,
         // not a branch: a run's time is its longest trajectory, where a

@@ -47,7 +47,7 @@ struct StepParams {
     double K0, V, dt, r_inj, theta_inj, tilt, polarity, ratio, delta0;
     double sqdt;          // sqrt(dt)
     double two_dt;        // 2 dt
-    double k2dt;          // 2 K0 dt T0 (the 1D models carry momenta in units of T0)
+    double sk2dt;         // sqrt(2 K0 dt T0) (the 1D models carry momenta in units of T0)
     double vdt;           // signed V dt as the model uses it (1D B-p: -V dt)
     double cp;            // (2/3) V dt with the same sign (momentum / energy loss)
     double csum;          // (4/3) V dt (1D F-p loss integral)

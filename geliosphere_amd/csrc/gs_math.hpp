@@ -3,8 +3,8 @@
 // The library versions of div, sqrt, log, exp, sin, cos round correctly or
 // handle every range (Payne-Hanek reduction, denormal scaling); the step
 // kernels only need ~1 ulp on known ranges.  These versions:
-//   rcp, rsqrt, sqrt  hardware seed (v_rcp_f64 / v_rsq_f64) + one
-//                     third-order correction;
+//   rcp, rsqrt, sqrt, hardware seed (v_rcp_f64 / v_rsq_f64 / v_sqrt_f64)
+//   rsqrt4            + one third-order correction;
 //   log               exponent split, s = (m-1)/(m+1), 2s + 2s w R(w);
 //   exp               k ln2 + r reduction (two-part ln2), Taylor to r^13;
 //   atan              three-point argument reduction + odd polynomial;
@@ -89,6 +89,17 @@ __device__ __forceinline__ double sqrt_nn(double x) {
     const double g = x * y;
     const double e = fma(-g, y, 1.0);
     return fma(g, e * fma(0.375, e, 0.5), g);
+}
+
+// x^(-1/4) for normal x > 0: the seed sqrt(rsq(x)) from two hardware
+// seeds, then the series of (1 - e)^(-1/4) to third order, e = 1 - x q^4,
+// written in f = e/4 = 1/4 - (x/4) q^4: q (1 + f + 5/2 f^2).  (Each fused
+// multiply-add then has at most one non-inline constant.)
+__device__ __forceinline__ double rsqrt4(double x) {
+    const double q = __builtin_amdgcn_sqrt(__builtin_amdgcn_rsq(x));
+    const double q2 = q * q;
+    const double f = fma(-0.25 * x, q2 * q2, 0.25);
+    return fma(q, f * fma(2.5, f, 1.0), q);
 }
 
 // sqrt(x), 0 for x <= 0.

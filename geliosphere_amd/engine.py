@@ -22,7 +22,7 @@ ORDER_ID, ORDER_ENERGY = 0, 1
 
 DBG_RCP, DBG_RSQRT, DBG_SQRT, DBG_LOG, DBG_EXP, DBG_SIN_TURN, DBG_COS_TURN, DBG_LOG_U32, \
     DBG_RAW_RCP, DBG_RAW_RSQ, DBG_RAW_SQRT, DBG_SIN, DBG_COS, DBG_ATAN, DBG_ATAN_FRAC, \
-    DBG_SHEET_F, DBG_SHEET_G = range(17)
+    DBG_SHEET_F, DBG_SHEET_G, DBG_RSQRT4 = range(18)
 TWO_ARG = (DBG_ATAN_FRAC, DBG_SHEET_F, DBG_SHEET_G)
 
 

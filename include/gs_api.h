@@ -259,6 +259,7 @@ typedef enum gs_debug_fn {
     GS_DBG_SHEET_F = 15,  /* 2D neutral-sheet drift factor f(sy = x[i],
                              k = x[n + i]) = atan(arg) / alphaH; 2n values */
     GS_DBG_SHEET_G = 16,  /* the same pair's tan(alphaH) / (alphaH (1 + arg^2)) */
+    GS_DBG_RSQRT4 = 17,   /* x^(-1/4), x > 0 normal                    */
     GS_DBG_NORMALS = 100  /* internal: normal stream (gs_debug_normals) */
 } gs_debug_fn;
 

@@ -140,6 +140,9 @@ def test_device_math_accuracy(engine):
         y = engine.debug_math(fn, xs)
         assert max(ulps(a, f(x)) for a, x in zip(y, xs)) <= 1.0
     assert engine.debug_math(E.DBG_SQRT, [0.0]) == [0.0]
+    # x^(-1/4) (the 1D steps' E^(-1/2) from E^2): two seeds and one correction
+    q = engine.debug_math(E.DBG_RSQRT4, xs)
+    assert max(ulps(a, mpmath.mpf(x) ** mpmath.mpf(-0.25)) for a, x in zip(q, xs)) <= 2.0
 
 
 @pytest.mark.gpu
