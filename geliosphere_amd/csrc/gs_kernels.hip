@@ -136,6 +136,15 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
     uint64_t sim_lo = 0, sim_hi = 0;
     StepParams lp;
     if constexpr (kBatch) lp = StepParams{};
+    constexpr bool kVecParams = kBatch || Model::kLatencyBound;
+    if constexpr (!kBatch && Model::kLatencyBound) {
+        // A latency-bound run ends with a lone wave whose every step would
+        // wait for a scalar reload of the (register-starved) parameters; in
+        // vector registers they stay put.  The empty asm (it emits nothing)
+        // makes them vector values.
+        lp = static_cast<const StepParams &>(P0);
+        asm volatile("" : "+v"(lp.cp), "+v"(lp.csum), "+v"(lp.sk2dt), "+v"(lp.vdt));
+    }
 
     Model m;
     uint64_t pid = 0;
@@ -151,7 +160,7 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
         else stage_event(stage, O, pid, t_inj, t_exit, r, theta, sumac);
     };
     auto do_step = [&](const double *z) -> bool {
-        if constexpr (kBatch) return m.step(lp, z, bm.trig, emit);
+        if constexpr (kVecParams) return m.step(lp, z, bm.trig, emit);
         else return m.step(P0, z, bm.trig, emit);
     };
 
