@@ -292,6 +292,8 @@ constexpr double kInvT0 = 1.0 / kT0;
 struct Bp {
     static constexpr int kNormals = 1;
     static constexpr bool kLatencyBound = false;
+    static constexpr bool kVecParams = false;
+    __device__ static void to_vgprs(StepParams &) {}
     static constexpr double kE2Exit = ((200.0 + kT0) * kInvT0) * ((200.0 + kT0) * kInvT0);
     double r, P, w, t_inj;
     __device__ __forceinline__ void save(double *s) const { s[0] = r; s[1] = P; s[2] = w; s[3] = t_inj; }
@@ -354,8 +356,15 @@ struct Bp {
 struct Fp {
     static constexpr int kNormals = 1;
     // A run lasts as long as its longest trajectory (a lone wave at the end),
-    // so per-step latency, not issue slots, sets its time.
+    // so per-step latency, not issue slots, sets its time.  That wave would
+    // wait for a scalar reload of the (register-starved) parameters in every
+    // step; in vector registers they stay put.  The empty asm emits nothing:
+    // it makes them vector values.
     static constexpr bool kLatencyBound = true;
+    static constexpr bool kVecParams = true;
+    __device__ static void to_vgprs(StepParams &p) {
+        asm volatile("" : "+v"(p.cp), "+v"(p.csum), "+v"(p.sk2dt), "+v"(p.vdt));
+    }
     static constexpr double kE2Cross = ((100.0 + kT0) * kInvT0) * ((100.0 + kT0) * kInvT0);
     double r, P, beta, w, sumac, t_inj;
     __device__ __forceinline__ void save(double *s) const {
@@ -413,6 +422,8 @@ struct Fp {
 struct Sp {
     static constexpr int kNormals = 2;
     static constexpr bool kLatencyBound = false;
+    static constexpr bool kVecParams = false;
+    __device__ static void to_vgprs(StepParams &) {}
     double r, theta, T, t_inj;
     __device__ __forceinline__ void save(double *s) const { s[0] = r; s[1] = theta; s[2] = T; s[3] = t_inj; }
     __device__ __forceinline__ void load(const double *s) { r = s[0]; theta = s[1]; T = s[2]; t_inj = s[3]; }
@@ -493,6 +504,8 @@ struct Sp {
 struct Ge {
     static constexpr int kNormals = 3;
     static constexpr bool kLatencyBound = false;
+    static constexpr bool kVecParams = false;
+    __device__ static void to_vgprs(StepParams &) {}
     double r, theta, T, t_inj;
     __device__ __forceinline__ void save(double *s) const { s[0] = r; s[1] = theta; s[2] = T; s[3] = t_inj; }
     __device__ __forceinline__ void load(const double *s) { r = s[0]; theta = s[1]; T = s[2]; t_inj = s[3]; }

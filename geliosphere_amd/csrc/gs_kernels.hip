@@ -136,14 +136,12 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
     uint64_t sim_lo = 0, sim_hi = 0;
     StepParams lp;
     if constexpr (kBatch) lp = StepParams{};
-    constexpr bool kVecParams = kBatch || Model::kLatencyBound;
-    if constexpr (!kBatch && Model::kLatencyBound) {
-        // A latency-bound run ends with a lone wave whose every step would
-        // wait for a scalar reload of the (register-starved) parameters; in
-        // vector registers they stay put.  The empty asm (it emits nothing)
-        // makes them vector values.
+    // Plain runs of models with Model::kVecParams hold the step parameters in
+    // vector registers (Model::to_vgprs), not scalar ones: see the models.
+    constexpr bool kVecParams = kBatch || Model::kVecParams;
+    if constexpr (!kBatch && Model::kVecParams) {
         lp = static_cast<const StepParams &>(P0);
-        asm volatile("" : "+v"(lp.cp), "+v"(lp.csum), "+v"(lp.sk2dt), "+v"(lp.vdt));
+        Model::to_vgprs(lp);
     }
 
     Model m;
