@@ -50,7 +50,8 @@ def parse():
                     help="models run per step (the headline workload is bp+fp)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
-    ap.add_argument("--batch-sims", type=int, default=64, help="simulations in the batched leg (0: skip)")
+    ap.add_argument("--batch-sims", type=int, default=256,
+                    help="simulations in the batched leg (0: skip); 256 as in the batch benchmark")
     ap.add_argument("--batch-scale", type=float, default=0.01,
                     help="particle counts of the batched leg relative to the batch sampler's 1e5-1e7")
     ap.add_argument("--no-surrogate", action="store_true")
@@ -102,6 +103,12 @@ def cpu_baseline(budget_s):
     return {"value": steps / t_total, "unit": "particle-steps/s", "cores": threads, "kind": "port",
             "sample": "1D B-p, %d particles (log-uniform 0.01-100 GeV, dt=500 s), %d steps in %.1f s"
                       % (sample_n, steps, t_total)}
+
+
+def progress(msg):
+    """One line on stderr when a leg of the run ends (stdout carries only the
+    result line)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
 
 
 def batch_leg(eng, n_sims, scale, seed=2024):
@@ -244,13 +251,17 @@ def main():
             "longest_particle_steps": {abi.MODEL_NAMES[m]: max(r[5] for r in runs if r[0] == m) for m in models},
         },
     }
+    progress("plain MC legs done: %.4g particle-steps/s" % out["value"])
     if a.batch_sims > 0:
         out["detail"]["batched"] = batch_leg(eng, a.batch_sims, a.batch_scale)
+        progress("batched leg done: %.1fx sequential plain" % out["detail"]["batched"]["speedup"])
     if not a.no_surrogate:
         out["detail"]["surrogate"] = surrogate_leg(eng)
+        progress("surrogate leg done")
     eng.close()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
+        progress("CPU baseline done")
     print(json.dumps(out), flush=True)
 
 
