@@ -178,8 +178,8 @@ def main():
         calibrate_copy(a.calibrate_copy)
         return
     if a.gpus != 1:
-        print("bench.py runs on one GPU only: this engine has no multi-GPU path in this build "
-              "(particles shard by id range; see DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: the engine has no multi-GPU path "
+              "(a run would shard by particle id range, one range per GPU)", file=sys.stderr)
         sys.exit(2)
     from geliosphere_amd import abi
     from geliosphere_amd.engine import Engine

@@ -74,7 +74,8 @@ def injection_energy(p, pid):
 
 
 def default_threads():
-    """Host threads for the oracle: all cores, at most 16 (a GPU box's share)."""
+    """Host threads for the oracle: all cores, at most 16 (one GPU's share of
+    a multi-GPU host)."""
     return max(1, min(16, os.cpu_count() or 1))
 
 
