@@ -483,13 +483,13 @@ struct Sp {
         dth = fma(-Pm.polthdt * dc * (2.0 + g2), inv_r, dth);
         r = r + dr;
         theta = theta + dth;
-        if (theta < 0.0) theta = fabs(theta);
+        theta = fabs(theta);   // theta < 0 -> |theta| (also takes -0 to +0)
         if (theta > 2.0 * kPi)
             theta = theta - (2.0 * kPi);
         else if (theta > kPi)
             theta = (2.0 * kPi) - theta;
         T = T - dTkin;
-        if (r < 0.0) r = -r;
+        r = fabs(r);           // r < 0 -> -r
         if (r > 100.0) {
             emit(t_inj, T, r, theta, 0.0);
             return false;
@@ -614,12 +614,12 @@ struct Ge {
         r = r + dr;
         theta = theta + dth;
         T = T - dTkin;
-        if (theta < 0.0) theta = fabs(theta);
+        theta = fabs(theta);   // theta < 0 -> |theta| (also takes -0 to +0)
         if (theta > 2.0 * kPi)
             theta = theta - (2.0 * kPi);
         else if (theta > kPi)
             theta = (2.0 * kPi) - theta;
-        if (r < 0.0) r = -r;
+        r = fabs(r);           // r < 0 -> -r
         if (r > 100.0) {
             emit(t_inj, T, r, theta, 0.0);
             return false;

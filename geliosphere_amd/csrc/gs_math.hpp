@@ -250,10 +250,10 @@ __host__ __device__ inline void sheet_drift(double sy, double k, double &f, doub
     const double X = sy, ax = fabs(sy);
     const double Y = Ops::root(kPiD * (kPiD - 2.0 * sy));
     const double a0 = atan_frac<Ops>(ax, Y);
-    const double aH = kPi2Hi - ((X < 0.0 ? -a0 : a0) - kPi2Lo);
+    const double aH = kPi2Hi - (copysign(a0, X) - kPi2Lo);   // a0 >= 0 takes X's sign
     const double kY = fabs(k) * Y;
     const bool swap = kY > ax;
-    const double a1 = atan_frac<Ops>(swap ? ax : kY, swap ? kY : ax);
+    const double a1 = atan_frac<Ops>(fmin(ax, kY), fmax(ax, kY));   // the smaller over the larger
     const double at = swap ? kPi2Hi - (a1 - kPi2Lo) : a1;     // atan(|k| Y / |X|)
     const double D2 = Ops::fma3(X, X, kY * kY);
     const double R = Ops::inv(aH * D2);
