@@ -946,9 +946,9 @@ gs_status gs_host_math(int32_t fn, const double *x, double *y, uint64_t n) {
         case GS_DBG_SIN: fm::sincos_tab<H>(x[i], kBmTrigHost, s, c); y[i] = s; break;
         case GS_DBG_COS: fm::sincos_tab<H>(x[i], kBmTrigHost, s, c); y[i] = c; break;
         case GS_DBG_ATAN: y[i] = fm::atan_e<H>(x[i]); break;
-        case GS_DBG_ATAN_FRAC: y[i] = fm::atan_frac<H>(x[i], x[n + i]); break;
-        case GS_DBG_SHEET_F: fm::sheet_drift<H>(x[i], x[n + i], s, c); y[i] = s; break;
-        case GS_DBG_SHEET_G: fm::sheet_drift<H>(x[i], x[n + i], s, c); y[i] = c; break;
+        case GS_DBG_ATAN_FRAC: y[i] = fm::atan_frac<H>(x[i], x[n + i], fm::kAtanRed); break;
+        case GS_DBG_SHEET_F: fm::sheet_drift<H>(x[i], x[n + i], fm::kAtanRed, s, c); y[i] = s; break;
+        case GS_DBG_SHEET_G: fm::sheet_drift<H>(x[i], x[n + i], fm::kAtanRed, s, c); y[i] = c; break;
         default: return GS_ERR_UNSUPPORTED;   // rcp / rsqrt / sqrt / rsqrt4 seeds exist on the device only
         }
     }

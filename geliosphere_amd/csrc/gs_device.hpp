@@ -65,12 +65,17 @@ __device__ __forceinline__ void philox10(uint32_t &x0, uint32_t &x1, uint32_t &x
 // (load_bm_tables, then a barrier) and reads them from there.
 alignas(16) __constant__ double kBmTrigTable[fm::kBmTrigWords] = GS_BM_TRIG_TABLE;
 alignas(16) __constant__ double kBmLogTable[fm::kBmLogWords] = GS_BM_LOG_TABLE;
+// trig holds the sin/cos table followed by the atan reduction table
+// (fm::kAtanRed at trig + kBmTrigWords), so that a step reaches both
+// through one pointer.
 struct alignas(16) BmTables {   // entries are read as 16-byte pairs
-    double trig[fm::kBmTrigWords];
+    double trig[fm::kBmTrigWords + fm::kAtanRedWords];
     double log[fm::kBmLogWords];
 };
 __device__ __forceinline__ void load_bm_tables(BmTables *t) {
     for (int i = (int)threadIdx.x; i < fm::kBmTrigWords; i += (int)blockDim.x) t->trig[i] = kBmTrigTable[i];
+    for (int i = (int)threadIdx.x; i < fm::kAtanRedWords; i += (int)blockDim.x)
+        t->trig[fm::kBmTrigWords + i] = fm::kAtanRed[i];
     for (int i = (int)threadIdx.x; i < fm::kBmLogWords; i += (int)blockDim.x) t->log[i] = kBmLogTable[i];
 }
 
@@ -474,7 +479,7 @@ struct Sp {
         // drifts: sheet angle alphaM + 2 Larmor pi / (180 r), Larmor = (0.0225 / A) P r^2 / sq
         const double sy = fm::sind(fma(Rig * r * inv_sq, (0.0225 / kA) * (kPi / 90.0), Pm.tilt), trig);
         double f, g;
-        fm::sheet_drift<fm::DevOps>(sy, 1.0 - theta * (2.0 / kPi), f, g);
+        fm::sheet_drift<fm::DevOps>(sy, 1.0 - theta * (2.0 / kPi), trig + fm::kBmTrigWords, f, g);
         const double rbg = Rig * beta * r * gamma * inv_opg;
         const double dc = rbg * ((kKonvF * (2.0 / (3.0 * kA))) * f) * inv_opg;
         // (DriftR + DriftSheetR) dt, with fprime = -(2/pi) g
@@ -604,7 +609,7 @@ struct Ge {
         // angle alphaM + 2 Larmor pi / (180 r), Larmor = (0.0225 / A) P r^2 / sqrt(Cb)
         const double sy = fm::sind(fma(Rig * r * inv_sqCb, (0.0225 / kA) * (kPi / 90.0), Pm.tilt), trig);
         double f, g;
-        fm::sheet_drift<fm::DevOps>(sy, 1.0 - theta * (2.0 / kPi), f, g);
+        fm::sheet_drift<fm::DevOps>(sy, 1.0 - theta * (2.0 / kPi), trig + fm::kBmTrigWords, f, g);
         const double rbg = Rig * beta * r * gamma * inv_Cb;
         const double dc = rbg * ((kKonvF * (2.0 / (3.0 * kA))) * f) * inv_Cb;
         // (DriftR + DriftSheetR) dt, with fprime = -(2/pi) g

@@ -504,9 +504,9 @@ __global__ void debug_kernel(int fn, const double *x, double *y, uint64_t n, uin
     case GS_DBG_SIN: fm::sincosd(x[i], bm.trig, s, c); y[i] = s; break;
     case GS_DBG_COS: fm::sincosd(x[i], bm.trig, s, c); y[i] = c; break;
     case GS_DBG_ATAN: y[i] = fm::atand(x[i]); break;
-    case GS_DBG_ATAN_FRAC: y[i] = fm::atan_fracd(x[i], x[n + i]); break;
-    case GS_DBG_SHEET_F: fm::sheet_drift<fm::DevOps>(x[i], x[n + i], s, c); y[i] = s; break;
-    case GS_DBG_SHEET_G: fm::sheet_drift<fm::DevOps>(x[i], x[n + i], s, c); y[i] = c; break;
+    case GS_DBG_ATAN_FRAC: y[i] = fm::atan_fracd(x[i], x[n + i], bm.trig + fm::kBmTrigWords); break;
+    case GS_DBG_SHEET_F: fm::sheet_drift<fm::DevOps>(x[i], x[n + i], bm.trig + fm::kBmTrigWords, s, c); y[i] = s; break;
+    case GS_DBG_SHEET_G: fm::sheet_drift<fm::DevOps>(x[i], x[n + i], bm.trig + fm::kBmTrigWords, s, c); y[i] = c; break;
     case GS_DBG_RSQRT4: y[i] = fm::rsqrt4(x[i]); break;
     case GS_DBG_NORMALS:
         normal_block(pid, (uint32_t)(i >> 2), (uint32_t)seed, (uint32_t)(seed >> 32), bm, z);

@@ -198,6 +198,16 @@ constexpr double kPi8Hi = 0.39269908169872414, kPi8Lo = 1.5308084989341915e-17; 
 constexpr double kPi4Hi = 0.78539816339744828, kPi4Lo = 3.061616997868383e-17;    // pi/4
 constexpr double kPi2Hi = 1.5707963267948966, kPi2Lo = 6.123233995736766e-17;     // pi/2
 
+// A 16-byte-aligned pair of table doubles, read as one 128-bit load.
+struct alignas(16) Pair {
+    double a, b;
+};
+__host__ __device__ inline Pair load_pair(const double *p) {
+    Pair v;
+    __builtin_memcpy(&v, __builtin_assume_aligned(p, 16), sizeof v);
+    return v;
+}
+
 // atan x.  |x| > 1 uses pi/2 - atan(1/|x|); then a in [0, 1] is moved to
 // t = (a - c) / (1 + c a) with c in {0, tan(pi/8), 1}, so that
 // |t| <= tan(pi/16), and atan a = atan c + atan t.
@@ -218,19 +228,25 @@ __host__ __device__ inline double atan_e(double x) {
     return x < 0.0 ? -r : r;
 }
 
+// The three reduction intervals of atan_frac as a table: (c, atan c hi,
+// lo, 0) for c = 0, tan(pi/8), 1.  The kernels keep a copy in LDS after
+// the sin/cos table; a lane reads its interval's entry instead of selecting
+// three constants out of three.
+constexpr int kAtanRedWords = 16;   // 12 used; 16 keeps the next table 16-byte aligned
+constexpr double kAtanRed[kAtanRedWords] = {0.0,     0.0,    0.0,    0.0, kTanPi8, kPi8Hi, kPi8Lo, 0.0,
+                                            1.0,     kPi4Hi, kPi4Lo, 0.0, 0.0,     0.0,    0.0,    0.0};
+
 // atan(n / d) for 0 <= n <= d, d > 0, with a single reciprocal: the same
 // reduction in fraction form, t = (n - c d) / (d + c n).  The interval tests
 // compare n with c d (no quotient); near a boundary either side's t stays
-// within the polynomial's range.
+// within the polynomial's range.  `red` is kAtanRed (or a copy of it).
 template <class Ops>
-__host__ __device__ inline double atan_frac(double n, double d) {
-    const bool low = n < kTanPi16 * d;
-    const bool high = n >= kTan3Pi16 * d;
-    const double cp = high ? 1.0 : (low ? 0.0 : kTanPi8);
-    const double base_hi = high ? kPi4Hi : (low ? 0.0 : kPi8Hi);
-    const double base_lo = high ? kPi4Lo : (low ? 0.0 : kPi8Lo);
-    const double t = Ops::fma3(-cp, d, n) * Ops::inv(Ops::fma3(cp, n, d));
-    return base_hi + (atan_reduced<Ops>(t) + base_lo);
+__host__ __device__ inline double atan_frac(double n, double d, const double *red) {
+    const int k = (n >= kTanPi16 * d ? 1 : 0) + (n >= kTan3Pi16 * d ? 1 : 0);
+    const Pair cb = load_pair(red + 4 * k);   // c, atan c hi
+    const double base_lo = red[4 * k + 2];
+    const double t = Ops::fma3(-cb.a, d, n) * Ops::inv(Ops::fma3(cb.a, n, d));
+    return cb.b + (atan_reduced<Ops>(t) + base_lo);
 }
 
 // Neutral-sheet drift factors of the 2D models (SolarPropLikeCpuModel.cpp:
@@ -245,15 +261,15 @@ __host__ __device__ inline double atan_frac(double n, double d) {
 // atan_frac reciprocals and one of alphaH (X^2 + k^2 Y^2) for both results.
 // At sy = 0 this gives the limits f = +-(pi/2)/alphaH, g = 0.
 template <class Ops>
-__host__ __device__ inline void sheet_drift(double sy, double k, double &f, double &g) {
+__host__ __device__ inline void sheet_drift(double sy, double k, const double *red, double &f, double &g) {
     constexpr double kPiD = 3.141592653589793;
     const double X = sy, ax = fabs(sy);
     const double Y = Ops::root(kPiD * (kPiD - 2.0 * sy));
-    const double a0 = atan_frac<Ops>(ax, Y);
+    const double a0 = atan_frac<Ops>(ax, Y, red);
     const double aH = kPi2Hi - (copysign(a0, X) - kPi2Lo);   // a0 >= 0 takes X's sign
     const double kY = fabs(k) * Y;
     const bool swap = kY > ax;
-    const double a1 = atan_frac<Ops>(fmin(ax, kY), fmax(ax, kY));   // the smaller over the larger
+    const double a1 = atan_frac<Ops>(fmin(ax, kY), fmax(ax, kY), red);   // the smaller over the larger
     const double at = swap ? kPi2Hi - (a1 - kPi2Lo) : a1;     // atan(|k| Y / |X|)
     const double D2 = Ops::fma3(X, X, kY * kY);
     const double R = Ops::inv(aH * D2);
@@ -262,16 +278,6 @@ __host__ __device__ inline void sheet_drift(double sy, double k, double &f, doub
 }
 
 // ------------------------------------------------------------ Box-Muller
-// A 16-byte-aligned pair of table doubles, read as one 128-bit load.
-struct alignas(16) Pair {
-    double a, b;
-};
-__host__ __device__ inline Pair load_pair(const double *p) {
-    Pair v;
-    __builtin_memcpy(&v, __builtin_assume_aligned(p, 16), sizeof v);
-    return v;
-}
-
 constexpr int kBmTrigWords = 512;   // (sin, cos)(2 pi j/256), j = 0..255
 constexpr int kBmLogWords = 512;    // 128 x (-2 invc, 2 ln invc hi, lo, 0)
 constexpr uint64_t kLogOff = 0x3fe5f00000000000ull;   // bits of 0.685546875
@@ -378,7 +384,9 @@ __device__ __forceinline__ double sind(double x, const double *trig) {
     return s;
 }
 __device__ __forceinline__ double atand(double x) { return atan_e<DevOps>(x); }
-__device__ __forceinline__ double atan_fracd(double n, double d) { return atan_frac<DevOps>(n, d); }
+__device__ __forceinline__ double atan_fracd(double n, double d, const double *red) {
+    return atan_frac<DevOps>(n, d, red);
+}
 
 // 1/a, 1/b, 1/c, 1/d from one reciprocal of abcd (a few ulp each; the
 // product must stay a normal number).
