@@ -457,7 +457,7 @@ struct Sp {
         const double inv_sq = sq * inv_opg;
         const double inv_opg15 = inv_opg * inv_sq;
         const double inv_opg25 = inv_opg * inv_opg15;
-        const double Rc = (Rig < 0.1) ? 0.1 : Rig;
+        const double Rc = fmax(Rig, 0.1);   // P below 0.1 GV counts as 0.1
         const double kbc = Pm.ge_k0c * beta;   // K0 beta (5/3.4)/3
         // K_par = K0 beta max(P, 0.1) (5/3.4) r^2 / (3 sqrt(1 + gamma^2))
         const double Kpar = kbc * Rc * r2 * inv_sq;
@@ -489,10 +489,10 @@ struct Sp {
         r = r + dr;
         theta = theta + dth;
         theta = fabs(theta);   // theta < 0 -> |theta| (also takes -0 to +0)
-        if (theta > 2.0 * kPi)
-            theta = theta - (2.0 * kPi);
-        else if (theta > kPi)
-            theta = (2.0 * kPi) - theta;
+        if (theta > kPi) {
+            asm volatile("");   // emits nothing; a pole crossing is rare: no selects on the common path
+            theta = theta > 2.0 * kPi ? theta - (2.0 * kPi) : (2.0 * kPi) - theta;
+        }
         T = T - dTkin;
         r = fabs(r);           // r < 0 -> -r
         if (r > 100.0) {
@@ -550,7 +550,11 @@ struct Ge {
         const double beta = Rig * inv_E;
         const bool polar = (theta < Pm.th_lo) || (theta > Pm.th_hi);
         const bool band = (theta > Pm.th_lo) && (theta < Pm.th_hi);
-        const double deltarh = polar ? 0.003 * (1.0 / kRh) : Pm.delta0_rh * inv_s;   // delta / rh
+        double deltarh = Pm.delta0_rh * inv_s;   // delta / rh
+        if (polar) {
+            asm volatile("");   // emits nothing; the polar caps are rare: no selects on the common path
+            deltarh = 0.003 * (1.0 / kRh);
+        }
         const double deltarh2 = deltarh * deltarh;
         const double gamma = r * Pm.omega_v * s;
         const double g2 = gamma * gamma;
@@ -562,7 +566,7 @@ struct Ge {
         const double inv_sqCb = sqCb * inv_Cb;
         const double inv_Cb15 = inv_Cb * inv_sqCb;
         const double inv_Cb25 = inv_Cb * inv_Cb15;
-        const double Rc = (Rig < 0.1) ? 0.1 : Rig;
+        const double Rc = fmax(Rig, 0.1);   // P below 0.1 GV counts as 0.1
         const double kbc = Pm.ge_k0c * beta;          // C K0 beta
         const double Kpar = kbc * Rc * r2 * inv_sqCb;
         const double Kper = Pm.ratio * Kpar;
@@ -573,7 +577,8 @@ struct Ge {
         // 1/Ktt and sqrt(2 Ktt dt) from one rsqrt
         const double rK = fm::rsqrt(Ktt);
         const double inv_Ktt = rK * rK;
-        const double B11 = fm::sqrt_pos(Pm.two_dt * fma(Krr, Ktt, -(Krt * Krt)) * inv_Ktt);   // times sqrt(dt)
+        // times sqrt(dt); a rounding-negative argument counts as 0
+        const double B11 = fm::sqrt_nn(fmax(Pm.two_dt * fma(Krr, Ktt, -(Krt * Krt)) * inv_Ktt, 0.0));
         const double B12 = (Pm.sq2dt * Krt) * rK;
         const double B22 = (Pm.sq2dt * Ktt) * rK * inv_r;
         const double kbrc = kbc * Rig;                // C K0 beta P
@@ -620,10 +625,10 @@ struct Ge {
         theta = theta + dth;
         T = T - dTkin;
         theta = fabs(theta);   // theta < 0 -> |theta| (also takes -0 to +0)
-        if (theta > 2.0 * kPi)
-            theta = theta - (2.0 * kPi);
-        else if (theta > kPi)
-            theta = (2.0 * kPi) - theta;
+        if (theta > kPi) {
+            asm volatile("");   // emits nothing; a pole crossing is rare: no selects on the common path
+            theta = theta > 2.0 * kPi ? theta - (2.0 * kPi) : (2.0 * kPi) - theta;
+        }
         r = fabs(r);           // r < 0 -> -r
         if (r > 100.0) {
             emit(t_inj, T, r, theta, 0.0);
