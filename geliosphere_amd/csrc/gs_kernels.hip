@@ -22,7 +22,23 @@ namespace gs {
 // region is full; the host then runs the simulation again with more room).
 __device__ __forceinline__ void emit_event(const DevOut &O, uint64_t pid, uint32_t sim, double t_inj, double t_exit,
                                         double r, double theta, double sumac) {
-    const unsigned long long i = atomicAdd((unsigned long long *)&O.n_events[sim], 1ull);
+    // The lanes emitting together nearly always share the simulation: then
+    // the first of them reserves the slots of all with one atomic (one
+    // counter per simulation takes every exit of a 1D batch otherwise).
+    const uint64_t lanes = __ballot(1);
+    const uint32_t s0 = __builtin_amdgcn_readfirstlane(sim);
+    unsigned long long i;
+    if (__ballot(sim != s0) == 0) {
+        const int lane = (int)(threadIdx.x & (kWave - 1));
+        const int leader = __ffsll((unsigned long long)lanes) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd((unsigned long long *)&O.n_events[s0], (unsigned long long)__popcll(lanes));
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)base, leader);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(base >> 32), leader);
+        i = (((unsigned long long)hi << 32) | lo) + (unsigned long long)__popcll(lanes & ((1ull << lane) - 1));
+    } else {
+        i = atomicAdd((unsigned long long *)&O.n_events[sim], 1ull);
+    }
     if (i < O.ev_cap[sim]) {
         // Streaming stores: a record is written once and read by the fold
         // pass, so it need not occupy L2 (and a lone record does not make
