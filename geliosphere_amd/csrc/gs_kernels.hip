@@ -55,7 +55,7 @@ __device__ __forceinline__ void emit_event(const DevOut &O, uint64_t pid, uint32
     }
 }
 
-// Event staging of a plain run: each wave appends its records to its own
+// Event staging: each wave appends its records to its own
 // LDS area (slot from an LDS counter) and, at a cycle boundary once at least
 // kStageFlush are staged, writes them out together: the wave reserves a
 // contiguous range of the event region and its 64 lanes store consecutive
@@ -71,8 +71,9 @@ struct Stage {
     uint32_t count[kBlockThreads / kWave];
 };
 
-__device__ __forceinline__ void stage_event(Stage *st, const DevOut &O, uint64_t pid, double t_inj, double t_exit,
-                                            double r, double theta, double sumac) {
+// `sim` is the record's simulation within the launch (0 for plain runs).
+__device__ __forceinline__ void stage_event(Stage *st, const DevOut &O, uint64_t pid, uint32_t sim, double t_inj,
+                                            double t_exit, double r, double theta, double sumac) {
     const int w = (int)(threadIdx.x / kWave);
     const uint32_t slot = atomicAdd(&st->count[w], 1u);
     if (slot < (uint32_t)kStage) {
@@ -83,25 +84,44 @@ __device__ __forceinline__ void stage_event(Stage *st, const DevOut &O, uint64_t
         d[3] = fm::double_to_bits(theta);
         d[4] = fm::double_to_bits(sumac);
         d[5] = pid;
-        d[6] = 0;
+        d[6] = sim;
         d[7] = 0;
     } else {
-        emit_event(O, pid, 0, t_inj, t_exit, r, theta, sumac);
+        emit_event(O, pid, sim, t_inj, t_exit, r, theta, sumac);
     }
 }
 
-// Called by the whole wave (uniform control flow).
+// Called by the whole wave (uniform control flow).  In a batch the staged
+// records normally share one simulation: they then go out as one range of
+// its region; otherwise each goes out on its own.
+template <bool kBatch>
 __device__ __forceinline__ void stage_flush(Stage *st, const DevOut &O, uint32_t min_count) {
     const int w = (int)(threadIdx.x / kWave), lane = (int)(threadIdx.x & (kWave - 1));
     __builtin_amdgcn_wave_barrier();
     uint32_t c = st->count[w];
     if (c > (uint32_t)kStage) c = kStage;
     if (c < min_count || c == 0) return;
+    uint32_t sim = 0;
+    if constexpr (kBatch) {
+        sim = (uint32_t)st->words[w][6];   // record 0's simulation
+        const uint32_t mine = (uint32_t)lane < c ? (uint32_t)st->words[w][lane * kRecWords + 6] : sim;
+        if (__ballot(mine != sim) != 0) {
+            if ((uint32_t)lane < c) {
+                const uint64_t *d = &st->words[w][lane * kRecWords];
+                emit_event(O, d[5], mine, fm::bits_to_double(d[0]), fm::bits_to_double(d[1]),
+                           fm::bits_to_double(d[2]), fm::bits_to_double(d[3]), fm::bits_to_double(d[4]));
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 0) st->count[w] = 0;
+            __builtin_amdgcn_wave_barrier();
+            return;
+        }
+    }
     unsigned long long base = 0;
-    if (lane == 0) base = atomicAdd((unsigned long long *)O.n_events, (unsigned long long)c);
+    if (lane == 0) base = atomicAdd((unsigned long long *)&O.n_events[sim], (unsigned long long)c);
     base = __shfl(base, 0);
-    const uint64_t cap = O.ev_cap[0];
-    uint64_t *g = (uint64_t *)(O.events + O.ev_off[0]);
+    const uint64_t cap = O.ev_cap[sim];
+    uint64_t *g = (uint64_t *)(O.events + O.ev_off[sim]);
     for (uint32_t i = (uint32_t)lane; i < c * kRecWords; i += kWave) {
         const uint64_t rec = base + i / kRecWords;
         if (rec < cap) __builtin_nontemporal_store(st->words[w][i], &g[base * kRecWords + i]);
@@ -170,8 +190,7 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
     uint32_t my_particles = 0, my_nan = 0, my_cap = 0, cycles = 0, my_max = 0;
 
     auto emit = [&](double t_inj, double t_exit, double r, double theta, double sumac) {
-        if constexpr (kBatch) emit_event(O, pid, sim, t_inj, t_exit, r, theta, sumac);
-        else stage_event(stage, O, pid, t_inj, t_exit, r, theta, sumac);
+        stage_event(stage, O, pid, sim, t_inj, t_exit, r, theta, sumac);
     };
     auto do_step = [&](const double *z) -> bool {
         if constexpr (kVecParams) return m.step(lp, z, bm.trig, emit);
@@ -282,8 +301,8 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
                     }
                 }
             }
-            stage_flush(stage, O, kStageFlush);
         }
+        stage_flush<kBatch>(stage, O, kStageFlush);
         if (__ballot(active) == 0) {
             if (exhausted) break;
             continue;
@@ -389,7 +408,7 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
         my_max = steps > my_max ? steps : my_max;
         if (isnan(m.r)) ++my_nan;
     }
-    if constexpr (!kBatch) stage_flush(stage, O, 1);
+    stage_flush<kBatch>(stage, O, 1);
     if constexpr (kBatch) {
         if (my_particles) flush_sim_stats(O, sim, my_steps, my_particles, my_nan, my_cap);
         if (lane == 0) atomicAdd((unsigned long long *)O.cycles, (unsigned long long)cycles);
@@ -467,11 +486,13 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
                                                               const uint64_t *__restrict__ offsets,
                                                               const DevOut O) {
     __shared__ uint64_t off[kBatchMaxSims + 1];
+    __shared__ Stage stage;
     __shared__ BmTables bm;
     for (uint32_t i = threadIdx.x; i <= n_sims; i += blockDim.x) off[i] = offsets[i];
+    if (threadIdx.x < kBlockThreads / kWave) stage.count[threadIdx.x] = 0;
     load_bm_tables(&bm);
     __syncthreads();
-    run_lanes<Model, true>(sims, n_sims, off, nullptr, nullptr, bm, O);
+    run_lanes<Model, true>(sims, n_sims, off, nullptr, &stage, bm, O);
 }
 
 // Histogram pass: workgroup (x, s) folds a strided share of simulation s's
