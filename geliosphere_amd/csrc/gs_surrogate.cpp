@@ -243,7 +243,7 @@ gs_status gs_surrogate_default_config(gs_surrogate_config *c) {
     if (!c) return GS_ERR_INVALID_ARG;
     std::memset(c, 0, sizeof *c);
     c->n_members = 4;
-    c->n_inputs = 7;
+    c->n_inputs = 8;
     c->n_outputs = 30;
     c->hidden = 256;
     c->n_hidden_layers = 4;

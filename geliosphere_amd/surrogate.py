@@ -3,9 +3,10 @@ include/gs_surrogate.h) and the host logic around it: features and targets
 of a simulation, data set generation with the batched engine, training,
 prediction with Monte Carlo fallback.
 
-Inputs (7): the model folded to two flags (2D?, and forward-vs-backward for
+Inputs (8): the model folded to two flags (2D?, and forward-vs-backward for
 1D / Geliosphere-vs-SolarProp for 2D), log10 K0 [cm^2/s], V [km/s] (1D) or
-tilt [deg] (2D), polarity, injection r [AU], injection theta [deg].
+tilt [deg] (2D), polarity, injection r [AU], injection theta [deg], log10 dt
+[s] (the 1D F-p spectrum depends on the time step).
 Outputs (30): per energy bin ln(mean event weight / J_LIS(T_c)) with the
 power-law LIS J = (T + 0.94)^-2.78 at the bin's geometric centre, i.e. the
 modulated-to-LIS ratio of the spectrum the engine's histogram estimates.
@@ -22,7 +23,7 @@ from .engine import lib as _core_lib, GsError
 
 K0_CONV = abi.K0_CM2_TO_AU2
 V_CONV = abi.V_KMS_TO_AUS
-N_IN, N_OUT = 7, 30
+N_IN, N_OUT = 8, 30
 FALLBACK_STD, FALLBACK_DOMAIN = 0x1, 0x2
 
 
@@ -180,6 +181,9 @@ class Surrogate:
 
     def load(self, path):
         d = np.load(path, allow_pickle=False)
+        if d["mean"].shape != (self.cfg.n_inputs,):
+            raise ValueError("%s holds a network of %d inputs, this ensemble has %d"
+                             % (path, d["mean"].shape[0], self.cfg.n_inputs))
         self.set_params(d["params"])
         self.set_normalization(d["mean"], d["sd"], d["lo"], d["hi"])
 
@@ -187,12 +191,12 @@ class Surrogate:
 # ------------------------------------------------------------------ features and targets
 
 def features(p):
-    """The 7 inputs of a parameter block."""
+    """The 8 inputs of a parameter block."""
     two_d = p.model in (abi.MODEL_2D_SOLARPROP, abi.MODEL_2D_GELIOSPHERE)
     sub = 1.0 if p.model in (abi.MODEL_1D_BP, abi.MODEL_2D_GELIOSPHERE) else 0.0
     v_or_tilt = math.degrees(p.tilt) if two_d else p.V / V_CONV
     return [1.0 if two_d else 0.0, sub, math.log10(p.K0 / K0_CONV), v_or_tilt,
-            p.polarity if two_d else 1.0, p.r_inj, math.degrees(p.theta_inj)]
+            p.polarity if two_d else 1.0, p.r_inj, math.degrees(p.theta_inj), math.log10(p.dt)]
 
 
 def bin_centres(p):

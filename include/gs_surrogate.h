@@ -32,7 +32,7 @@ typedef struct gs_surrogate gs_surrogate;
 
 typedef struct gs_surrogate_config {
     uint32_t n_members;        /* ensemble size (default 4)                  */
-    uint32_t n_inputs;         /* <= GS_SG_MAX_INPUTS (default 7)            */
+    uint32_t n_inputs;         /* <= GS_SG_MAX_INPUTS (default 8)            */
     uint32_t n_outputs;        /* <= GS_SG_MAX_OUTPUTS (default 30)          */
     uint32_t hidden;           /* multiple of 64 (default 256)               */
     uint32_t n_hidden_layers;  /* >= 1 (default 4)                           */

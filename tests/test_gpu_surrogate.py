@@ -4,6 +4,7 @@ FP64 oracle network, and its training, determinism and fallback logic.
 import numpy as np
 import pytest
 
+from geliosphere_amd.surrogate import N_IN as NI
 from oracle import surrogate_ref as R
 
 pytestmark = pytest.mark.gpu
@@ -39,11 +40,11 @@ def test_gradients_match_fp64_oracle():
     rng = np.random.default_rng(11)
     s = make(seed=5)
     n = 200
-    x = rng.normal(0, 1, (n, 7)).astype(np.float32)
+    x = rng.normal(0, 1, (n, NI)).astype(np.float32)
     y = rng.normal(-1, 0.5, (n, 30)).astype(np.float32)
     w = rng.uniform(0, 2, (n, 30)).astype(np.float32)
     w[rng.uniform(size=w.shape) < 0.2] = 0.0
-    s.set_normalization(np.zeros(7), np.ones(7), np.full(7, -10.0), np.full(7, 10.0))
+    s.set_normalization(np.zeros(NI), np.ones(NI), np.full(NI, -10.0), np.full(NI, 10.0))
     g, loss = s.gradients(x, y, w)
     pm, _ = s.layout()
     emu = reference_grads(s, x, y, w, emulate=True)
@@ -64,12 +65,12 @@ def test_gradients_match_fp64_oracle():
 def test_predict_matches_oracle_forward():
     rng = np.random.default_rng(12)
     s = make(seed=9)
-    s.set_normalization(np.zeros(7), np.ones(7), np.full(7, -10.0), np.full(7, 10.0))
-    x = rng.normal(0, 1, (300, 7)).astype(np.float32)
+    s.set_normalization(np.zeros(NI), np.ones(NI), np.full(NI, -10.0), np.full(NI, 10.0))
+    x = rng.normal(0, 1, (300, NI)).astype(np.float32)
     mean, sd, fb = s.predict(x)
     pm, lay = s.layout()
     P = s.get_params().astype(np.float64)
-    xp = R.pad_inputs(x.astype(np.float64), np.zeros(7), np.ones(7))
+    xp = R.pad_inputs(x.astype(np.float64), np.zeros(NI), np.ones(NI))
     outs = np.array([R.forward(R.unpack(P[m * pm:(m + 1) * pm], lay), xp, emulate_bf16=True)[0][:, :30]
                      for m in range(s.cfg.n_members)])
     assert np.max(np.abs(mean - outs.mean(axis=0))) < 2e-3
@@ -77,7 +78,7 @@ def test_predict_matches_oracle_forward():
 
 
 def synthetic(n, rng):
-    x = rng.uniform(-1, 1, (n, 7)).astype(np.float32)
+    x = rng.uniform(-1, 1, (n, NI)).astype(np.float32)
     b = np.arange(30) / 29.0
     y = (np.sin(2.0 * x[:, :1] + b) * 0.5 + x[:, 1:2] * x[:, 2:3] * b - 0.3 * x[:, 3:4] ** 2).astype(np.float32)
     return x, y, np.ones_like(y)
@@ -113,8 +114,8 @@ def test_fallback_flags_and_save_load(tmp_path):
     x, y, w = synthetic(2048, rng)
     s = make(seed=4, batch=256, std_limit=1e9)
     s.train(x, y, w, 2)
-    inside = np.zeros((1, 7), dtype=np.float32)
-    outside = np.full((1, 7), 5.0, dtype=np.float32)
+    inside = np.zeros((1, NI), dtype=np.float32)
+    outside = np.full((1, NI), 5.0, dtype=np.float32)
     _, _, fb = s.predict(np.vstack([inside, outside]))
     assert fb[0] == 0 and fb[1] & 0x2
     path = tmp_path / "sg.npz"

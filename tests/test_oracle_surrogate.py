@@ -66,6 +66,8 @@ def test_features_of_parameter_blocks():
     f = features(p)
     assert f[:2] == [0.0, 1.0] and f[2] == pytest.approx(math.log10(2e22), abs=1e-12)
     assert f[3] == pytest.approx(550.0, rel=1e-12) and f[4] == 1.0 and f[5] == 3.0 and f[6] == pytest.approx(40.0)
+    assert len(f) == 8 and f[7] == pytest.approx(math.log10(p.dt), abs=1e-12)
+    assert features(abi.make_params(abi.MODEL_1D_FP, dt=50.0))[7] == pytest.approx(math.log10(50.0), abs=1e-12)
     q = abi.make_params(abi.MODEL_2D_SOLARPROP, tilt_deg=33.0, polarity=-1.0)
     g = features(q)
     assert g[:2] == [1.0, 0.0] and g[3] == pytest.approx(33.0) and g[4] == -1.0
