@@ -41,6 +41,7 @@ struct gs_surrogate {
     float mu[GS_SG_MAX_INPUTS], sd[GS_SG_MAX_INPUTS], lo[GS_SG_MAX_INPUTS], hi[GS_SG_MAX_INPUTS];
     // work buffers, grown on demand
     std::vector<DevBuf> bufs;
+    std::vector<float> h_sd;           // predict: host spread when the caller wants flags only
     std::string err;
 };
 
@@ -67,7 +68,7 @@ gs_status fail(gs_surrogate *s, gs_status st, const char *fmt, ...) {
 // Named work buffers (index into s->bufs).
 enum Buf {
     B_X = 0,      // data set inputs, bf16 [n][kPad]
-    B_XF,         // staging fp32 [n][kPad]
+    B_XF,         // staging fp32 [n][n_inputs]
     B_Y,          // data set targets fp32 [n][kPad]
     B_W,          // data set weights fp32 [n][kPad]
     B_IDX,        // minibatch rows uint32 [E][B]
@@ -199,16 +200,13 @@ gs_status backward(gs_surrogate *s, const __bf16 *X, size_t xstride, int rows, c
 
 // fp32 rows [n][n_inputs] -> standardised, padded bf16 rows [n][kPad] on the device.
 gs_status upload_inputs(gs_surrogate *s, const float *x, uint32_t n, __bf16 **out) {
-    std::vector<float> h((size_t)n * kPad, 0.0f);
     const uint32_t ni = s->cfg.n_inputs;
-    for (uint32_t r = 0; r < n; ++r)
-        for (uint32_t c = 0; c < ni; ++c) h[(size_t)r * kPad + c] = (x[(size_t)r * ni + c] - s->mu[c]) / s->sd[c];
     float *xf;
     __bf16 *xb;
-    SG_NEED(s, B_XF, h.size(), xf);
-    SG_NEED(s, B_X, h.size(), xb);
-    SG_TRY(s, hipMemcpyAsync(xf, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, s->stream));
-    SG_TRY(s, sg_launch_to_bf16(xf, xb, h.size(), s->stream));
+    SG_NEED(s, B_XF, (size_t)n * ni, xf);
+    SG_NEED(s, B_X, (size_t)n * kPad, xb);
+    SG_TRY(s, hipMemcpyAsync(xf, x, (size_t)n * ni * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    SG_TRY(s, sg_launch_standardise(xf, (int)n, (int)ni, kPad, s->mu, s->sd, xb, s->stream));
     SG_TRY(s, hipStreamSynchronize(s->stream));
     *out = xb;
     return GS_OK;
@@ -560,7 +558,6 @@ gs_status gs_surrogate_predict(gs_surrogate *s, const float *x, uint32_t n, floa
     const uint32_t no = s->cfg.n_outputs, ni = s->cfg.n_inputs;
     const int E = s->E, H = (int)s->cfg.hidden;
     const uint32_t chunk = 65536;
-    std::vector<float> hm, hs;
     for (uint32_t r0 = 0; r0 < n; r0 += chunk) {
         const uint32_t rows = std::min(chunk, n - r0);
         __bf16 *X, *act;
@@ -574,13 +571,16 @@ gs_status gs_surrogate_predict(gs_surrogate *s, const float *x, uint32_t n, floa
         st = forward(s, X, 0, (int)rows, nullptr, act, out);
         if (st != GS_OK) return st;
         SG_TRY(s, sg_launch_stats(out, E, (int)rows, kPad, (int)no, (size_t)rows * kPad, dm, ds, s->stream));
-        hm.resize((size_t)rows * no);
-        hs.resize((size_t)rows * no);
-        SG_TRY(s, hipMemcpyAsync(hm.data(), dm, hm.size() * sizeof(float), hipMemcpyDeviceToHost, s->stream));
-        SG_TRY(s, hipMemcpyAsync(hs.data(), ds, hs.size() * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+        // Results go straight into the caller's rows.
+        const size_t bytes = (size_t)rows * no * sizeof(float);
+        float *hs = sd ? sd + (size_t)r0 * no : nullptr;
+        if (!hs && fallback) {
+            s->h_sd.resize((size_t)rows * no);
+            hs = s->h_sd.data();
+        }
+        if (mean) SG_TRY(s, hipMemcpyAsync(mean + (size_t)r0 * no, dm, bytes, hipMemcpyDeviceToHost, s->stream));
+        if (hs) SG_TRY(s, hipMemcpyAsync(hs, ds, bytes, hipMemcpyDeviceToHost, s->stream));
         SG_TRY(s, hipStreamSynchronize(s->stream));
-        if (mean) std::copy(hm.begin(), hm.end(), mean + (size_t)r0 * no);
-        if (sd) std::copy(hs.begin(), hs.end(), sd + (size_t)r0 * no);
         if (fallback)
             for (uint32_t r = 0; r < rows; ++r) {
                 uint8_t f = 0;

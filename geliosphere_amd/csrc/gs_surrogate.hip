@@ -204,6 +204,24 @@ __global__ void to_bf16_kernel(const float *x, __bf16 *y, size_t n) {
         y[i] = (__bf16)x[i];
 }
 
+struct StdArgs {
+    float mu[32], sd[32];
+};
+
+// One thread per padded element.  The subtraction and the IEEE-rounded
+// division are those the host would do, so the rows equal a host
+// standardisation followed by the bf16 conversion.
+__global__ void standardise_kernel(const float *x, int n, int ni, int ldo, StdArgs a, __bf16 *y) {
+    const size_t total = (size_t)n * ldo;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t r = i / ldo;
+        const int c = (int)(i - r * ldo);
+        float v = 0.0f;
+        if (c < ni) v = (x[r * ni + c] - a.mu[c]) / a.sd[c];
+        y[i] = (__bf16)v;
+    }
+}
+
 // Mean and (population) standard deviation over the members of [E][n][ldy]
 // outputs, first `cols` columns.
 __global__ void ensemble_stats_kernel(const float *Y, int E, int n, int ldy, int cols, size_t strideY, float *mean,
@@ -283,6 +301,18 @@ hipError_t sg_launch_adam(float *P, const float *G, float *Mo, float *Ve, __bf16
 
 hipError_t sg_launch_to_bf16(const float *x, __bf16 *y, size_t n, hipStream_t s) {
     hipLaunchKernelGGL(to_bf16_kernel, dim3(1024), dim3(256), 0, s, x, y, n);
+    return hipGetLastError();
+}
+
+hipError_t sg_launch_standardise(const float *x, int n, int ni, int ldo, const float *mu, const float *sd, __bf16 *y,
+                                 hipStream_t s) {
+    if (ni < 1 || ni > 32 || ni > ldo) return hipErrorInvalidValue;
+    StdArgs a{};
+    for (int c = 0; c < ni; ++c) {
+        a.mu[c] = mu[c];
+        a.sd[c] = sd[c];
+    }
+    hipLaunchKernelGGL(standardise_kernel, dim3(1024), dim3(256), 0, s, x, n, ni, ldo, a, y);
     return hipGetLastError();
 }
 

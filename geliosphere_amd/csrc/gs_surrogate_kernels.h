@@ -37,5 +37,9 @@ hipError_t sg_launch_gather(const __bf16 *X, const float *Y, const float *W, con
 hipError_t sg_launch_adam(float *P, const float *G, float *Mo, float *Ve, __bf16 *Pb, size_t n, float lr, float b1,
                           float b2, float eps, float wd, float c1, float c2, hipStream_t s);
 hipError_t sg_launch_to_bf16(const float *x, __bf16 *y, size_t n, hipStream_t s);
+// Standardised, zero-padded bf16 rows [n][ldo] from fp32 rows [n][ni]:
+// y = bf16((x - mu) / sd), with ni <= 32.
+hipError_t sg_launch_standardise(const float *x, int n, int ni, int ldo, const float *mu, const float *sd, __bf16 *y,
+                                 hipStream_t s);
 hipError_t sg_launch_stats(const float *Y, int E, int n, int ldy, int cols, size_t strideY, float *mean, float *sd,
                            hipStream_t s);

@@ -2,6 +2,8 @@
 
     python -m geliosphere_amd.surrogate_eval --data A.npz [B.npz ...]
         [--epochs 1500] [--save WEIGHTS.npz] [--json OUT.json]
+    python -m geliosphere_amd.surrogate_eval --data A.npz --load WEIGHTS.npz
+        (measures a saved ensemble; trains only if --epochs is given too)
 
 Reports, per model and overall, on the held-out evaluation spectra (made
 with 10x more particles than the training spectra):
@@ -98,16 +100,21 @@ def inference_rate(sg, n=65536, repeats=5):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--data", nargs="+", required=True)
-    ap.add_argument("--epochs", type=int, default=1500)
+    ap.add_argument("--epochs", type=int, default=None, help="default 1500, or 0 with --load")
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--save", default="")
+    ap.add_argument("--load", default="", help="start from a saved ensemble")
     ap.add_argument("--json", default="")
     a = ap.parse_args(argv)
     d = load(a.data)
     print("training spectra %d, evaluation spectra %d" % (len(d["x"]), len(d["ex"])), flush=True)
     sg = surrogate.Surrogate(0, batch=a.batch, lr=a.lr, seed=a.seed)
+    if a.epochs is None:
+        a.epochs = 0 if a.load else 1500
+    if a.load:
+        sg.load(a.load)
     t0 = time.perf_counter()
     done, loss = 0, None
     while done < a.epochs:
