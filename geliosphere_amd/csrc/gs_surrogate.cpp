@@ -139,6 +139,35 @@ gs_status forward(gs_surrogate *s, const __bf16 *X, size_t xstride, int rows, fl
     return GS_OK;
 }
 
+// Inference forward pass into Out [E][rows][kPad]: the fused kernel when the
+// network has its shape (hidden 256, at most kMlpMaxLayers weight layers),
+// otherwise the layer-by-layer GEMMs through the activation buffer.
+bool fused_shape(const gs_surrogate *s) {
+    return s->cfg.hidden == 256 && s->L >= 2 && s->L <= kMlpMaxLayers && s->dims[0] == 32 && s->dims[s->L] == 32;
+}
+
+gs_status forward_infer(gs_surrogate *s, const __bf16 *X, int rows, float *Out) {
+    if (fused_shape(s)) {
+        MlpArgs g{};
+        g.X = X;
+        g.rows = rows;
+        g.L = s->L;
+        g.Pb = s->Pb;
+        g.P = s->P;
+        g.Pm = s->Pm;
+        for (int l = 0; l < s->L; ++l) {
+            g.woff[l] = s->woff[l];
+            g.boff[l] = s->boff[l];
+        }
+        g.Out = Out;
+        SG_TRY(s, sg_launch_mlp_forward(g, s->E, s->stream));
+        return GS_OK;
+    }
+    __bf16 *act;
+    SG_NEED(s, B_ACT, (size_t)(s->L - 1) * s->E * rows * s->cfg.hidden, act);
+    return forward(s, X, 0, rows, nullptr, act, Out);
+}
+
 // Backward pass from dY (fp32 and bf16 [E][rows][kPad]) into s->G.
 gs_status backward(gs_surrogate *s, const __bf16 *X, size_t xstride, int rows, const float *Zbuf,
                    const __bf16 *Act, const float *dYf, const __bf16 *dYb) {
@@ -556,19 +585,18 @@ gs_status gs_surrogate_predict(gs_surrogate *s, const float *x, uint32_t n, floa
     if (n == 0) return GS_OK;
     SG_TRY(s, hipSetDevice(s->device));
     const uint32_t no = s->cfg.n_outputs, ni = s->cfg.n_inputs;
-    const int E = s->E, H = (int)s->cfg.hidden;
+    const int E = s->E;
     const uint32_t chunk = 65536;
     for (uint32_t r0 = 0; r0 < n; r0 += chunk) {
         const uint32_t rows = std::min(chunk, n - r0);
-        __bf16 *X, *act;
+        __bf16 *X;
         float *out, *dm, *ds;
         gs_status st = upload_inputs(s, x + (size_t)r0 * ni, rows, &X);
         if (st != GS_OK) return st;
-        SG_NEED(s, B_ACT, (size_t)(s->L - 1) * E * rows * H, act);
         SG_NEED(s, B_OUT, (size_t)E * rows * kPad, out);
         SG_NEED(s, B_MEAN, (size_t)rows * no, dm);
         SG_NEED(s, B_SD, (size_t)rows * no, ds);
-        st = forward(s, X, 0, (int)rows, nullptr, act, out);
+        st = forward_infer(s, X, (int)rows, out);
         if (st != GS_OK) return st;
         SG_TRY(s, sg_launch_stats(out, E, (int)rows, kPad, (int)no, (size_t)rows * kPad, dm, ds, s->stream));
         // Results go straight into the caller's rows.

@@ -115,6 +115,95 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
             }
 }
 
+// Fused inference of one member over a tile of FR rows: every layer runs in
+// one workgroup, with the bf16 activations held in LDS between layers instead
+// of making a round trip through HBM.  Hidden width 256, padded input and
+// output widths 32.  Each output element sees the same MFMA sequence as in
+// gemm_kernel (one 16x16x32 step per 32 values of k, k ascending) and the
+// same epilogue, so the outputs equal the layer-by-layer forward pass.
+constexpr int FR = 64, FH = 256, FLD = FH + 8;
+
+// One hidden layer in place: every wave reads the whole tile, then (after a
+// barrier) overwrites its own 64 columns with the activations.
+template <int K>
+__device__ __forceinline__ void mlp_hidden_layer(__bf16 *buf, const __bf16 *W, const float *bias, int lane,
+                                                 int wave) {
+    const int fr = lane & 15, fk = (lane >> 4) * 8, n0 = wave * 64;
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        bf16x8 a[4], b[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = *(const bf16x8 *)(W + (size_t)(n0 + 16 * j + fr) * K + k0 + fk);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = *(const bf16x8 *)(buf + (16 * i + fr) * FLD + k0 + fk);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int n = n0 + 16 * j + fr;
+        const float bn = bias[n];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int m = 16 * i + 4 * (lane >> 4) + v;
+                buf[m * FLD + n] = (__bf16)silu(acc[i][j][v] + bn);
+            }
+    }
+}
+
+__global__ void __launch_bounds__(256, 4) mlp_forward_kernel(MlpArgs g) {
+    __shared__ __bf16 act[FR * FLD];
+    const int e = blockIdx.y, row0 = blockIdx.x * FR;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const __bf16 *Pb = g.Pb + (size_t)e * g.Pm;
+    const float *P = g.P + (size_t)e * g.Pm;
+    {   // input tile: FR rows x 32 bf16, 8 per thread, zero beyond the last row
+        const int r = t >> 2, c = (t & 3) * 8;
+        bf16x8 v = {};
+        if (row0 + r < g.rows) v = *(const bf16x8 *)(g.X + (size_t)(row0 + r) * 32 + c);
+        *(bf16x8 *)(act + r * FLD + c) = v;
+    }
+    __syncthreads();
+    mlp_hidden_layer<32>(act, Pb + g.woff[0], P + g.boff[0], lane, wave);
+    __syncthreads();
+    for (int l = 1; l < g.L - 1; ++l) {
+        mlp_hidden_layer<FH>(act, Pb + g.woff[l], P + g.boff[l], lane, wave);
+        __syncthreads();
+    }
+    // output layer, 32 padded columns: wave w owns rows 16w..16w+15
+    const int l = g.L - 1, fr = lane & 15, fk = (lane >> 4) * 8;
+    const __bf16 *W = Pb + g.woff[l];
+    const float *bias = P + g.boff[l];
+    f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+    for (int k0 = 0; k0 < FH; k0 += 32) {
+        const bf16x8 a = *(const bf16x8 *)(act + (16 * wave + fr) * FLD + k0 + fk);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const bf16x8 b = *(const bf16x8 *)(W + (size_t)(16 * j + fr) * FH + k0 + fk);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int m = 16 * wave + 4 * (lane >> 4) + v, n = 16 * j + fr;
+            if (row0 + m < g.rows) g.Out[((size_t)e * g.rows + row0 + m) * 32 + n] = acc[j][v] + bias[n];
+        }
+}
+
 // Column sums of a [rows x cols] fp32 matrix per member (bias gradients).
 // One thread per column, rows walked in order: deterministic.
 __global__ void colsum_kernel(const float *X, int rows, int cols, size_t strideX, float *out, size_t strideOut) {
@@ -268,6 +357,12 @@ hipError_t sg_launch_gemm(bool a_col, bool b_col, int epi, const GemmArgs &g, in
     }
     if (a_col && b_col) return launch_gemm_t<true, true, EPI_STORE>(g, members, s);
     return hipErrorInvalidValue;
+}
+
+hipError_t sg_launch_mlp_forward(const MlpArgs &g, int members, hipStream_t s) {
+    if (g.rows <= 0 || members <= 0 || g.L < 2 || g.L > kMlpMaxLayers) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(mlp_forward_kernel, dim3((unsigned)((g.rows + FR - 1) / FR), members), dim3(256), 0, s, g);
+    return hipGetLastError();
 }
 
 hipError_t sg_launch_colsum(const float *X, int rows, int cols, size_t strideX, float *out, size_t strideOut,

@@ -26,7 +26,21 @@ struct GemmArgs {
     __bf16 *Cb;
 };
 
+// Fused inference (all layers of a member per 64-row tile, activations in LDS):
+// hidden width 256, input and output padded to 32, 2..kMlpMaxLayers weight layers.
+constexpr int kMlpMaxLayers = 8;
+struct MlpArgs {
+    const __bf16 *X;                    // [rows][32] standardised inputs
+    int rows, L;                        // L weight layers
+    const __bf16 *Pb;                   // bf16 parameters, member stride Pm
+    const float *P;                     // fp32 parameters (biases)
+    size_t Pm;
+    size_t woff[kMlpMaxLayers], boff[kMlpMaxLayers];
+    float *Out;                         // [members][rows][32]
+};
+
 hipError_t sg_launch_gemm(bool a_col, bool b_col, int epi, const GemmArgs &g, int members, hipStream_t s);
+hipError_t sg_launch_mlp_forward(const MlpArgs &g, int members, hipStream_t s);
 hipError_t sg_launch_colsum(const float *X, int rows, int cols, size_t strideX, float *out, size_t strideOut,
                             int members, hipStream_t s);
 hipError_t sg_launch_loss(const float *Y, const float *T, const float *W, int n, size_t strideY, size_t strideT,

@@ -62,6 +62,23 @@ def test_gradients_match_fp64_oracle():
         assert cos > 0.999
 
 
+def test_fused_inference_equals_training_forward():
+    """predict runs the fused all-layer kernel; the training path runs one
+    GEMM per layer.  With one member the prediction is that member's output,
+    so taking it as the training target must give a loss and a gradient of
+    exactly zero: the two forward passes agree bit for bit.  300 rows leave a
+    partial 64-row tile."""
+    rng = np.random.default_rng(13)
+    s = make(seed=4, n_members=1)
+    s.set_normalization(np.zeros(NI), np.ones(NI), np.full(NI, -10.0), np.full(NI, 10.0))
+    x = rng.normal(0, 1, (300, NI)).astype(np.float32)
+    mean, _, _ = s.predict(x)
+    assert np.all(np.isfinite(mean)) and np.abs(mean).max() > 0
+    g, loss = s.gradients(x, mean, np.ones_like(mean))
+    assert loss[0] == 0.0
+    assert np.abs(g).max() == 0.0
+
+
 def test_predict_matches_oracle_forward():
     rng = np.random.default_rng(12)
     s = make(seed=9)
