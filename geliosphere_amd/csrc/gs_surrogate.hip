@@ -121,7 +121,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
 // output widths 32.  Each output element sees the same MFMA sequence as in
 // gemm_kernel (one 16x16x32 step per 32 values of k, k ascending) and the
 // same epilogue, so the outputs equal the layer-by-layer forward pass.
-constexpr int FR = 64, FH = 256, FLD = FH + 8;
+constexpr int FR = 128, FH = 256, FLD = FH + 8, FRT = FR / 16;
 
 // One hidden layer in place: every wave reads the whole tile, then (after a
 // barrier) overwrites its own 64 columns with the activations.
@@ -129,23 +129,23 @@ template <int K>
 __device__ __forceinline__ void mlp_hidden_layer(__bf16 *buf, const __bf16 *W, const float *bias, int lane,
                                                  int wave) {
     const int fr = lane & 15, fk = (lane >> 4) * 8, n0 = wave * 64;
-    f32x4 acc[4][4];
+    f32x4 acc[FRT][4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < FRT; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 2
     for (int k0 = 0; k0 < K; k0 += 32) {
-        bf16x8 a[4], b[4];
+        bf16x8 b[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) b[j] = *(const bf16x8 *)(W + (size_t)(n0 + 16 * j + fr) * K + k0 + fk);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) a[i] = *(const bf16x8 *)(buf + (16 * i + fr) * FLD + k0 + fk);
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < FRT; ++i) {
+            const bf16x8 a = *(const bf16x8 *)(buf + (16 * i + fr) * FLD + k0 + fk);
 #pragma unroll
             for (int j = 0; j < 4; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[i][j], 0, 0, 0);
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -153,7 +153,7 @@ __device__ __forceinline__ void mlp_hidden_layer(__bf16 *buf, const __bf16 *W, c
         const int n = n0 + 16 * j + fr;
         const float bn = bias[n];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < FRT; ++i)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int m = 16 * i + 4 * (lane >> 4) + v;
@@ -162,14 +162,14 @@ __device__ __forceinline__ void mlp_hidden_layer(__bf16 *buf, const __bf16 *W, c
     }
 }
 
-__global__ void __launch_bounds__(256, 4) mlp_forward_kernel(MlpArgs g) {
+__global__ void __launch_bounds__(256, 2) mlp_forward_kernel(MlpArgs g) {
     __shared__ __bf16 act[FR * FLD];
     const int e = blockIdx.y, row0 = blockIdx.x * FR;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const __bf16 *Pb = g.Pb + (size_t)e * g.Pm;
     const float *P = g.P + (size_t)e * g.Pm;
-    {   // input tile: FR rows x 32 bf16, 8 per thread, zero beyond the last row
-        const int r = t >> 2, c = (t & 3) * 8;
+    for (int q = t; q < FR * 4; q += 256) {   // input tile: FR rows x 32 bf16, zero beyond the last row
+        const int r = q >> 2, c = (q & 3) * 8;
         bf16x8 v = {};
         if (row0 + r < g.rows) v = *(const bf16x8 *)(g.X + (size_t)(row0 + r) * 32 + c);
         *(bf16x8 *)(act + r * FLD + c) = v;
@@ -181,27 +181,35 @@ __global__ void __launch_bounds__(256, 4) mlp_forward_kernel(MlpArgs g) {
         mlp_hidden_layer<FH>(act, Pb + g.woff[l], P + g.boff[l], lane, wave);
         __syncthreads();
     }
-    // output layer, 32 padded columns: wave w owns rows 16w..16w+15
-    const int l = g.L - 1, fr = lane & 15, fk = (lane >> 4) * 8;
+    // output layer, 32 padded columns: wave w owns rows FR/4 * w .. FR/4 * (w + 1) - 1
+    constexpr int ORT = FRT / 4;
+    const int l = g.L - 1, fr = lane & 15, fk = (lane >> 4) * 8, r0 = wave * (FR / 4);
     const __bf16 *W = Pb + g.woff[l];
     const float *bias = P + g.boff[l];
-    f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+    f32x4 acc[ORT][2];
+#pragma unroll
+    for (int i = 0; i < ORT; ++i) acc[i][0] = acc[i][1] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int k0 = 0; k0 < FH; k0 += 32) {
-        const bf16x8 a = *(const bf16x8 *)(act + (16 * wave + fr) * FLD + k0 + fk);
+        bf16x8 b[2];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const bf16x8 b = *(const bf16x8 *)(W + (size_t)(16 * j + fr) * FH + k0 + fk);
-            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
+        for (int j = 0; j < 2; ++j) b[j] = *(const bf16x8 *)(W + (size_t)(16 * j + fr) * FH + k0 + fk);
+#pragma unroll
+        for (int i = 0; i < ORT; ++i) {
+            const bf16x8 a = *(const bf16x8 *)(act + (r0 + 16 * i + fr) * FLD + k0 + fk);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[i][j], 0, 0, 0);
         }
     }
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int i = 0; i < ORT; ++i)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int m = 16 * wave + 4 * (lane >> 4) + v, n = 16 * j + fr;
-            if (row0 + m < g.rows) g.Out[((size_t)e * g.rows + row0 + m) * 32 + n] = acc[j][v] + bias[n];
-        }
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int m = r0 + 16 * i + 4 * (lane >> 4) + v, n = 16 * j + fr;
+                if (row0 + m < g.rows) g.Out[((size_t)e * g.rows + row0 + m) * 32 + n] = acc[i][j][v] + bias[n];
+            }
 }
 
 // Column sums of a [rows x cols] fp32 matrix per member (bias gradients).

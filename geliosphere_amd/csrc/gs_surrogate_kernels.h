@@ -26,7 +26,7 @@ struct GemmArgs {
     __bf16 *Cb;
 };
 
-// Fused inference (all layers of a member per 64-row tile, activations in LDS):
+// Fused inference (all layers of a member per 128-row tile, activations in LDS):
 // hidden width 256, input and output padded to 32, 2..kMlpMaxLayers weight layers.
 constexpr int kMlpMaxLayers = 8;
 struct MlpArgs {

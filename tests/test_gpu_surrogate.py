@@ -67,7 +67,7 @@ def test_fused_inference_equals_training_forward():
     GEMM per layer.  With one member the prediction is that member's output,
     so taking it as the training target must give a loss and a gradient of
     exactly zero: the two forward passes agree bit for bit.  300 rows leave a
-    partial 64-row tile."""
+    partial 128-row tile."""
     rng = np.random.default_rng(13)
     s = make(seed=4, n_members=1)
     s.set_normalization(np.zeros(NI), np.ones(NI), np.full(NI, -10.0), np.full(NI, 10.0))
