@@ -41,7 +41,6 @@ struct gs_surrogate {
     float mu[GS_SG_MAX_INPUTS], sd[GS_SG_MAX_INPUTS], lo[GS_SG_MAX_INPUTS], hi[GS_SG_MAX_INPUTS];
     // work buffers, grown on demand
     std::vector<DevBuf> bufs;
-    std::vector<float> h_sd;           // predict: host spread when the caller wants flags only
     std::string err;
 };
 
@@ -80,6 +79,7 @@ enum Buf {
     B_DZF0, B_DZB0, B_DZF1, B_DZB1,   // hidden gradients, ping-pong
     B_WSUM, B_LOSS,
     B_MEAN, B_SD,
+    B_FLAGS,      // predict: fallback flags uint8 [n]
     B_COUNT_
 };
 
@@ -599,28 +599,21 @@ gs_status gs_surrogate_predict(gs_surrogate *s, const float *x, uint32_t n, floa
         st = forward_infer(s, X, (int)rows, out);
         if (st != GS_OK) return st;
         SG_TRY(s, sg_launch_stats(out, E, (int)rows, kPad, (int)no, (size_t)rows * kPad, dm, ds, s->stream));
-        // Results go straight into the caller's rows.
+        // Flags on the device (from the raw inputs still in B_XF); results go
+        // straight into the caller's rows.
         const size_t bytes = (size_t)rows * no * sizeof(float);
-        float *hs = sd ? sd + (size_t)r0 * no : nullptr;
-        if (!hs && fallback) {
-            s->h_sd.resize((size_t)rows * no);
-            hs = s->h_sd.data();
+        if (fallback) {
+            const float *xraw;
+            uint8_t *fl;
+            SG_NEED(s, B_XF, (size_t)rows * ni, xraw);
+            SG_NEED(s, B_FLAGS, (size_t)rows, fl);
+            SG_TRY(s, sg_launch_flags(ds, xraw, (int)rows, (int)no, (int)ni, s->cfg.std_outputs, s->cfg.std_limit,
+                                      s->lo, s->hi, fl, s->stream));
+            SG_TRY(s, hipMemcpyAsync(fallback + r0, fl, rows, hipMemcpyDeviceToHost, s->stream));
         }
         if (mean) SG_TRY(s, hipMemcpyAsync(mean + (size_t)r0 * no, dm, bytes, hipMemcpyDeviceToHost, s->stream));
-        if (hs) SG_TRY(s, hipMemcpyAsync(hs, ds, bytes, hipMemcpyDeviceToHost, s->stream));
+        if (sd) SG_TRY(s, hipMemcpyAsync(sd + (size_t)r0 * no, ds, bytes, hipMemcpyDeviceToHost, s->stream));
         SG_TRY(s, hipStreamSynchronize(s->stream));
-        if (fallback)
-            for (uint32_t r = 0; r < rows; ++r) {
-                uint8_t f = 0;
-                for (uint32_t c = 0; c < no; ++c)
-                    if (((s->cfg.std_outputs >> c) & 1u) && !(hs[(size_t)r * no + c] <= s->cfg.std_limit))
-                        f |= GS_SG_FALLBACK_STD;
-                for (uint32_t c = 0; c < ni; ++c) {
-                    const float v = x[(size_t)(r0 + r) * ni + c];
-                    if (!(v >= s->lo[c] && v <= s->hi[c])) f |= GS_SG_FALLBACK_DOMAIN;
-                }
-                fallback[r0 + r] = f;
-            }
     }
     return GS_OK;
 }

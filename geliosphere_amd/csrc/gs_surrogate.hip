@@ -319,6 +319,27 @@ __global__ void standardise_kernel(const float *x, int n, int ni, int ldo, StdAr
     }
 }
 
+struct FlagArgs {
+    float lo[32], hi[32];
+};
+
+// Fallback flags, one thread per query: bit 0x1 when an output selected by
+// `mask` has a spread that is not <= `limit` (NaN included), bit 0x2 when an
+// input lies outside [lo, hi] (NaN included).
+__global__ void fallback_flags_kernel(const float *sd, const float *x, int n, int no, int ni, uint32_t mask,
+                                      float limit, FlagArgs b, uint8_t *flags) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    uint32_t f = 0;
+    for (int c = 0; c < no; ++c)
+        if (((mask >> c) & 1u) && !(sd[(size_t)r * no + c] <= limit)) f |= 0x1u;
+    for (int c = 0; c < ni; ++c) {
+        const float v = x[(size_t)r * ni + c];
+        if (!(v >= b.lo[c] && v <= b.hi[c])) f |= 0x2u;
+    }
+    flags[r] = (uint8_t)f;
+}
+
 // Mean and (population) standard deviation over the members of [E][n][ldy]
 // outputs, first `cols` columns.
 __global__ void ensemble_stats_kernel(const float *Y, int E, int n, int ldy, int cols, size_t strideY, float *mean,
@@ -416,6 +437,19 @@ hipError_t sg_launch_standardise(const float *x, int n, int ni, int ldo, const f
         a.sd[c] = sd[c];
     }
     hipLaunchKernelGGL(standardise_kernel, dim3(1024), dim3(256), 0, s, x, n, ni, ldo, a, y);
+    return hipGetLastError();
+}
+
+hipError_t sg_launch_flags(const float *sd, const float *x, int n, int no, int ni, uint32_t mask, float limit,
+                           const float *lo, const float *hi, uint8_t *flags, hipStream_t s) {
+    if (n <= 0 || no < 0 || no > 32 || ni < 1 || ni > 32) return hipErrorInvalidValue;
+    FlagArgs b{};
+    for (int c = 0; c < ni; ++c) {
+        b.lo[c] = lo[c];
+        b.hi[c] = hi[c];
+    }
+    hipLaunchKernelGGL(fallback_flags_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sd, x, n, no, ni,
+                       mask, limit, b, flags);
     return hipGetLastError();
 }
 

@@ -55,5 +55,10 @@ hipError_t sg_launch_to_bf16(const float *x, __bf16 *y, size_t n, hipStream_t s)
 // y = bf16((x - mu) / sd), with ni <= 32.
 hipError_t sg_launch_standardise(const float *x, int n, int ni, int ldo, const float *mu, const float *sd, __bf16 *y,
                                  hipStream_t s);
+// Fallback flags per query from the ensemble spread sd [n][no] and the raw
+// inputs x [n][ni]: 0x1 spread above limit on an output of mask, 0x2 input
+// outside [lo, hi]; no, ni <= 32.
+hipError_t sg_launch_flags(const float *sd, const float *x, int n, int no, int ni, uint32_t mask, float limit,
+                           const float *lo, const float *hi, uint8_t *flags, hipStream_t s);
 hipError_t sg_launch_stats(const float *Y, int E, int n, int ldy, int cols, size_t strideY, float *mean, float *sd,
                            hipStream_t s);

@@ -163,15 +163,17 @@ class Surrogate:
                                                  loss.ctypes.data_as(ctypes.POINTER(ctypes.c_float))), "gradients")
         return g, loss
 
-    def predict(self, x):
+    def predict(self, x, want_sd=True):
+        """(mean, sd, fallback flags) of n queries; sd is None when not wanted,
+        which halves the bytes copied back."""
         x, px = _f32(x)
         n = x.shape[0]
         mean = np.zeros((n, self.cfg.n_outputs), dtype=np.float32)
-        sd = np.zeros_like(mean)
+        sd = np.zeros_like(mean) if want_sd else None
         fb = np.zeros(n, dtype=np.uint8)
+        psd = sd.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if want_sd else None
         self._check(lib().gs_surrogate_predict(self._h, px, n, mean.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
-                                               sd.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
-                                               fb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))), "predict")
+                                               psd, fb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))), "predict")
         return mean, sd, fb
 
     def save(self, path):
@@ -244,7 +246,7 @@ def query(surrogate, engine, sims):
     trusted, Monte Carlo (batched engine) where the ensemble disagrees or the
     query is outside the training box.  Returns (y [n][30], from_mc [n])."""
     x = np.array([features(p) for p in sims], dtype=np.float32)
-    mean, sd, fb = surrogate.predict(x)
+    mean, _, fb = surrogate.predict(x, want_sd=False)
     y = mean.astype(np.float64)
     redo = [i for i in range(len(sims)) if fb[i]]
     if redo:

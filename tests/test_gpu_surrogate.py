@@ -143,3 +143,17 @@ def test_fallback_flags_and_save_load(tmp_path):
     m2, sd2, fb2 = t.predict(x[:64])
     assert np.array_equal(m1, m2) and np.array_equal(sd1, sd2)
     assert np.all(fb2 & 0x1)          # std_limit 0: every query falls back
+    # without the spread the flags and means are the same
+    m3, sd3, fb3 = t.predict(x[:64], want_sd=False)
+    assert sd3 is None and np.array_equal(m3, m2) and np.array_equal(fb3, fb2)
+    # flags against the rule on the host, with a limit that about half the queries pass
+    sel = [c for c in range(30) if (s.cfg.std_outputs >> c) & 1]
+    u = make(seed=4, batch=256, std_limit=float(np.median(sd1[:, sel].max(axis=1))))
+    u.load(path)
+    q = np.vstack([x[:64], outside])
+    m4, sd4, fb4 = u.predict(q)
+    lo, hi = u.get_normalization()[2:]
+    mask = u.cfg.std_outputs
+    want = np.array([(0x1 if any((mask >> c) & 1 and not sd4[r, c] <= u.cfg.std_limit for c in range(30)) else 0) |
+                     (0x2 if not np.all((q[r] >= lo) & (q[r] <= hi)) else 0) for r in range(len(q))], dtype=np.uint8)
+    assert np.array_equal(fb4, want) and 0 < np.count_nonzero(fb4 & 0x1) < len(q)
