@@ -55,6 +55,9 @@ struct gs_ctx {
     std::vector<uint64_t> graph_key;
     hipGraphExec_t graph_exec = nullptr;
     uint64_t batch_event_budget = 1ull << 29;
+    // plain runs: particles per launch and event records held (0: room for all)
+    uint64_t plain_chunk = 1ull << 27;
+    uint64_t plain_event_cap = 0;
     // plain runs: scheduling order of a chunk's particles (gs_set_order)
     int32_t order_mode = GS_ORDER_ENERGY;
     uint32_t *d_order = nullptr;        // permutation, cap_order entries
@@ -92,7 +95,8 @@ gs_status fail(gs_ctx *ctx, gs_status st, const char *fmt, ...) {
                         "%s failed: %s", #call, hipGetErrorString(e_));                     \
     } while (0)
 
-size_t words_per_sim() { return 1 + GS_MAX_BINS + gs::kDigitsPerSim + gs::ST_COUNT_; }
+constexpr size_t kWordsPerSim = 1 + GS_MAX_BINS + gs::kDigitsPerSim + gs::ST_COUNT_;
+size_t words_per_sim() { return kWordsPerSim; }
 
 bool finite(double x) { return std::isfinite(x); }
 
@@ -597,6 +601,192 @@ gs_status run_sub_batch(gs_ctx *ctx, const gs_params *sims, const std::vector<ui
     return GS_OK;
 }
 
+// Device buffers of one plain run.  meta: [0] event offset (0), [1] event
+// capacity, [2] event count, [3] cycles, [4..5] hand-on counts,
+// [8 .. 8+ST_COUNT_) statistics saved before a chunk.
+// own = true: the context's buffers, which grow on demand (event region,
+// order permutation, hand-on records).  own = false: fixed buffers (events,
+// order_counts and order of cap_events entries); no exit records, no
+// hand-on, and chunks are cut to fit the fixed event region.
+struct PlainBufs {
+    uint64_t *words;
+    DevParams *params;
+    uint64_t *meta;
+    bool own;
+    gs::DevEvent *events;
+    uint64_t cap_events;
+    uint32_t *order_counts;
+    uint32_t *order;
+};
+
+// One plain run, as consecutive chunks of the id range.  A chunk whose events
+// overflow the event region runs again (trajectories are deterministic),
+// from the statistics saved before it: with more room (own buffers), or as
+// half as many particles (fixed buffers).
+gs_status run_plain_core(gs_ctx *ctx, const gs_params &pp, gs_spectrum *out, gs_exit_record *recs,
+                         uint64_t rec_cap, uint64_t *n_recs, const PlainBufs &B) {
+    const gs_params *p = &pp;
+    const auto t_start = std::chrono::steady_clock::now();
+    const bool want_recs = recs != nullptr && rec_cap > 0;
+    if (want_recs && !B.own) return fail(ctx, GS_ERR_INVALID_ARG, "exit records need the context's buffers");
+    Layout L = layout(B.words, 1);
+    HIP_TRY(ctx, gs_launch_zero(B.words, kWordsPerSim, ctx->stream));
+    const unsigned grid = grid_for(ctx, p->model, false);
+    std::vector<gs_exit_record> kept;   // records only (never with fixed buffers)
+    uint64_t total_events = 0;
+    double kernel_ms = 0.0;
+    const uint64_t N = p->n_particles;
+    DevOut O;
+    O.queue = L.queue; O.g_count = L.count; O.g_digits = L.digits; O.g_stats = L.stats;
+    O.events = nullptr;
+    O.ev_off = B.meta; O.ev_cap = B.meta + 1; O.n_events = B.meta + 2;
+    O.cycles = B.meta + 3;
+    O.order = nullptr;
+    O.spill = nullptr; O.spill_count = nullptr; O.spill_cap = 0;
+    O.resume = nullptr; O.resume_n = 0;
+    const uint64_t lanes = (uint64_t)grid * gs::kBlockThreads;
+    // largest chunk: 2^27 particles, or what the fixed event region holds
+    // (B-p and 2D models emit at most one event per particle)
+    const uint64_t chunk_max = B.own ? ctx->plain_chunk
+                                     : (p->model == GS_MODEL_1D_FP ? B.cap_events * 16 : B.cap_events);
+    // most event records a chunk may hold
+    const uint64_t ev_limit = B.own ? (ctx->plain_event_cap ? ctx->plain_event_cap : ~0ull) : B.cap_events;
+    DevParams hp;              // host copy of the chunk's parameters
+    uint64_t hm[2] = {0, 0};   // event offset and capacity
+    gs_status st = GS_OK;
+    uint64_t c0 = 0;
+    uint64_t size = std::min(chunk_max, N);
+    bool grown = false;   // own buffers: the current chunk's region was grown once
+    do {
+        gs_params q = *p;
+        q.first_particle = p->first_particle + c0;
+        q.n_particles = size;
+        hp = derive(q);
+        if (B.own) {
+            st = ensure_events(ctx, std::min(event_room(q.model, q.n_particles), ev_limit));
+            if (st != GS_OK) return st;
+        }
+        gs::DevEvent *events = B.own ? ctx->d_events : B.events;
+        const uint64_t cap = std::min(B.own ? ctx->cap_events : B.cap_events, ev_limit);
+        HIP_TRY(ctx, hipMemcpyAsync(B.params, &hp, sizeof(DevParams), hipMemcpyHostToDevice, ctx->stream));
+        const gs::OrderKey key = order_key(q, hp);
+        O.order = nullptr;
+        if (ctx->order_mode == GS_ORDER_ENERGY && key.nb > 0 && q.n_particles >= kOrderMinParticles) {
+            uint32_t *counts = B.order_counts, *perm = B.order;
+            if (B.own) {
+                st = ensure_order(ctx, q.n_particles);
+                if (st != GS_OK) return st;
+                counts = ctx->d_order_counts;
+                perm = ctx->d_order;
+            }
+            HIP_TRY(ctx, gs_launch_order(B.params, q.n_particles, key, counts, perm, ctx->stream));
+            O.order = perm;
+        }
+        // statistics before the chunk, restored if it has to run again
+        HIP_TRY(ctx, hipMemcpyAsync(B.meta + 8, L.stats, gs::ST_COUNT_ * sizeof(uint64_t),
+                                    hipMemcpyDeviceToDevice, ctx->stream));
+        hm[0] = 0;
+        hm[1] = cap;
+        HIP_TRY(ctx, hipMemcpyAsync(B.meta, hm, 2 * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(ctx, gs_launch_zero(L.queue, 1, ctx->stream));
+        HIP_TRY(ctx, gs_launch_zero(B.meta + 2, 4, ctx->stream));   // events, cycles, hand-on counts
+        O.events = events;
+        const bool compact = B.own && (q.model == GS_MODEL_1D_FP || q.n_particles <= kCompactPerLane * lanes);
+        if (compact) {
+            st = ensure_spill(ctx, lanes);   // at most one hand-on per lane and launch
+            if (st != GS_OK) return st;
+        }
+        O.resume = nullptr;
+        O.resume_n = 0;
+        O.spill = compact ? ctx->d_spill : nullptr;
+        O.spill_count = B.meta + 4;
+        O.spill_cap = compact ? ctx->cap_spill : 0;
+        HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        HIP_TRY(ctx, gs_launch_traj(q.model, false, B.params, 1, nullptr, O, grid, ctx->stream));
+        // continuations: the handed-on particles, packed into full waves
+        for (int round = 0, cur = 0; compact; ++round, cur ^= 1) {
+            uint64_t pending = 0;
+            HIP_TRY(ctx, hipMemcpyAsync(&pending, B.meta + 4 + cur, sizeof pending, hipMemcpyDeviceToHost,
+                                        ctx->stream));
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            pending = std::min(pending, O.spill_cap);   // slots beyond the cap were not handed on
+            if (pending == 0) break;
+            HIP_TRY(ctx, gs_launch_zero(L.queue, 1, ctx->stream));
+            HIP_TRY(ctx, gs_launch_zero(B.meta + 4 + (cur ^ 1), 1, ctx->stream));
+            O.resume = ctx->d_spill + (size_t)cur * ctx->cap_spill;
+            O.resume_n = pending;
+            O.spill = pending > kCompactMin && round + 1 < kCompactRounds
+                          ? ctx->d_spill + (size_t)(cur ^ 1) * ctx->cap_spill : nullptr;
+            O.spill_count = B.meta + 4 + (cur ^ 1);
+            const unsigned g2 = (unsigned)std::min<uint64_t>(grid, (pending + gs::kBlockThreads - 1) /
+                                                                      gs::kBlockThreads);
+            HIP_TRY(ctx, gs_launch_traj(q.model, false, B.params, 1, nullptr, O, g2, ctx->stream));
+        }
+        O.resume = nullptr;
+        O.spill = nullptr;
+        HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        uint64_t nev = 0;
+        HIP_TRY(ctx, hipMemcpyAsync(&nev, B.meta + 2, sizeof nev, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        float ms = 0.f;
+        HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        kernel_ms += ms;
+        if (nev > cap) {
+            // The region was too small (1D F-p crossings are unbounded): run
+            // the chunk again from the statistics saved before it, with room
+            // for its nev events where the limit allows, else as half as
+            // many particles.
+            HIP_TRY(ctx, hipMemcpyAsync(L.stats, B.meta + 8, gs::ST_COUNT_ * sizeof(uint64_t),
+                                        hipMemcpyDeviceToDevice, ctx->stream));
+            if (B.own && !grown && nev <= ev_limit) {
+                st = ensure_events(ctx, std::min(nev + nev / 8, ev_limit));
+                if (st != GS_OK) return st;
+                grown = true;
+                continue;
+            }
+            if (size == 1)
+                return fail(ctx, GS_ERR_OVERFLOW, "particle %llu has more than %llu events, the event capacity",
+                            (unsigned long long)q.first_particle, (unsigned long long)cap);
+            size /= 2;
+            grown = false;
+            continue;
+        }
+        HIP_TRY(ctx, gs_launch_fold(B.params, 1, O, fold_lds_bytes(q.n_bins, q.n_weights), fold_blocks(nev),
+                                    ctx->stream));
+        total_events += nev;
+        if (want_recs && nev > 0 && kept.size() < rec_cap) {
+            std::vector<gs::DevEvent> h(nev);
+            HIP_TRY(ctx, hipMemcpyAsync(h.data(), events, nev * sizeof(gs::DevEvent), hipMemcpyDeviceToHost,
+                                        ctx->stream));
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            std::vector<gs_exit_record> rs(nev);
+            for (uint64_t i = 0; i < nev; ++i) rs[i] = to_record(h[i]);
+            std::sort(rs.begin(), rs.end(), record_less);
+            for (uint64_t i = 0; i < nev && kept.size() < rec_cap; ++i) kept.push_back(rs[i]);
+        }
+        c0 += size;
+        size = std::min(chunk_max, N - c0);
+        grown = false;
+    } while (c0 < N);
+    uint64_t host[kWordsPerSim];
+    HIP_TRY(ctx, hipMemcpyAsync(host, B.words, sizeof host, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    Layout H = layout(host, 1);
+    H.stats[gs::ST_EVENTS] = total_events;
+    fill_spectrum(*p, H.count, H.digits, H.stats, out);
+    if (want_recs) std::copy(kept.begin(), kept.end(), recs);
+    if (n_recs) *n_recs = total_events;
+    ctx->stats.kernel_ms = kernel_ms;
+    ctx->stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    ctx->stats.lane_cycles = H.stats[gs::ST_CYCLES] * (uint64_t)gs::kWave * gs::kCycleSteps;
+    ctx->stats.max_particle_steps = H.stats[gs::ST_MAX_STEPS];
+    ctx->stats.active_cycles = H.stats[gs::ST_STEPS];
+    ctx->stats.grid_blocks = grid;
+    ctx->stats.block_threads = gs::kBlockThreads;
+    if (out->n_acc_overflow) return fail(ctx, GS_ERR_OVERFLOW, "weight accumulator overflow");
+    return GS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -757,140 +947,21 @@ gs_status gs_run_plain(gs_ctx *ctx, const gs_params *p, gs_spectrum *out, gs_exi
     if (!out) return fail(ctx, GS_ERR_INVALID_ARG, "out is NULL");
     std::string msg;
     if (!check_params(p, msg)) return fail(ctx, GS_ERR_INVALID_ARG, "invalid params: %s", msg.c_str());
-    const auto t_start = std::chrono::steady_clock::now();
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     gs_status st = ensure_sims(ctx, 1);
     if (st == GS_OK) st = ensure_params(ctx, 1);
-    if (st != GS_OK) return st;
     if (st == GS_OK) st = ensure_meta(ctx, 64);
     if (st != GS_OK) return st;
-    const bool want_recs = recs != nullptr && rec_cap > 0;
-    Layout L = layout(ctx->d_words, 1);
-    HIP_TRY(ctx, gs_launch_zero(ctx->d_words, words_per_sim(), ctx->stream));
-    const unsigned grid = grid_for(ctx, p->model, false);
-    std::vector<gs_exit_record> kept;
-    uint64_t total_events = 0;
-    double kernel_ms = 0.0;
-    const uint64_t N = p->n_particles;
-    // d_meta: [0] event offset (0), [1] event capacity, [2] event count, [3] cycles
-    DevOut O;
-    O.queue = L.queue; O.g_count = L.count; O.g_digits = L.digits; O.g_stats = L.stats;
-    O.ev_off = ctx->d_meta; O.ev_cap = ctx->d_meta + 1; O.n_events = ctx->d_meta + 2;
-    O.cycles = ctx->d_meta + 3;
-    O.order = nullptr;
-    O.spill = nullptr; O.spill_count = nullptr; O.spill_cap = 0;
-    O.resume = nullptr; O.resume_n = 0;
-    const uint64_t lanes = (uint64_t)grid * gs::kBlockThreads;
-    ctx->h_meta.assign(2, 0);
-    for (uint64_t c0 = 0; c0 < N || (N == 0 && c0 == 0); c0 += kMaxChunk) {
-        gs_params q = *p;
-        q.first_particle = p->first_particle + c0;
-        q.n_particles = std::min(kMaxChunk, N - c0);
-        ctx->h_params.assign(1, derive(q));
-        st = ensure_events(ctx, event_room(q.model, q.n_particles));
-        if (st != GS_OK) return st;
-        HIP_TRY(ctx, hipMemcpyAsync(ctx->d_params, ctx->h_params.data(), sizeof(DevParams),
-                                    hipMemcpyHostToDevice, ctx->stream));
-        const gs::OrderKey key = order_key(q, ctx->h_params[0]);
-        O.order = nullptr;
-        if (ctx->order_mode == GS_ORDER_ENERGY && key.nb > 0 && q.n_particles >= kOrderMinParticles) {
-            st = ensure_order(ctx, q.n_particles);
-            if (st != GS_OK) return st;
-            HIP_TRY(ctx, gs_launch_order(ctx->d_params, q.n_particles, key, ctx->d_order_counts, ctx->d_order,
-                                         ctx->stream));
-            O.order = ctx->d_order;
-        }
-        uint64_t nev = 0;
-        for (int attempt = 0; attempt < 2; ++attempt) {
-            ctx->h_meta[0] = 0;
-            ctx->h_meta[1] = ctx->cap_events;
-            HIP_TRY(ctx, hipMemcpyAsync(ctx->d_meta, ctx->h_meta.data(), 2 * sizeof(uint64_t),
-                                        hipMemcpyHostToDevice, ctx->stream));
-            HIP_TRY(ctx, gs_launch_zero(L.queue, 1, ctx->stream));
-            HIP_TRY(ctx, gs_launch_zero(ctx->d_meta + 2, 4, ctx->stream));   // events, cycles, spill counts
-            O.events = ctx->d_events;
-            const bool compact = q.model == GS_MODEL_1D_FP || q.n_particles <= kCompactPerLane * lanes;
-            if (compact) {
-                st = ensure_spill(ctx, lanes);   // at most one hand-on per lane and launch
-                if (st != GS_OK) return st;
-            }
-            O.resume = nullptr;
-            O.resume_n = 0;
-            O.spill = compact ? ctx->d_spill : nullptr;
-            O.spill_count = ctx->d_meta + 4;
-            O.spill_cap = ctx->cap_spill;
-            HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-            HIP_TRY(ctx, gs_launch_traj(q.model, false, ctx->d_params, 1, nullptr, O, grid, ctx->stream));
-            // continuations: the handed-on particles, packed into full waves
-            for (int round = 0, cur = 0; compact; ++round, cur ^= 1) {
-                uint64_t pending = 0;
-                HIP_TRY(ctx, hipMemcpyAsync(&pending, ctx->d_meta + 4 + cur, sizeof pending, hipMemcpyDeviceToHost,
-                                            ctx->stream));
-                HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-                pending = std::min(pending, O.spill_cap);   // slots beyond the cap were not handed on
-                if (pending == 0) break;
-                HIP_TRY(ctx, gs_launch_zero(L.queue, 1, ctx->stream));
-                HIP_TRY(ctx, gs_launch_zero(ctx->d_meta + 4 + (cur ^ 1), 1, ctx->stream));
-                O.resume = ctx->d_spill + (size_t)cur * ctx->cap_spill;
-                O.resume_n = pending;
-                O.spill = pending > kCompactMin && round + 1 < kCompactRounds
-                              ? ctx->d_spill + (size_t)(cur ^ 1) * ctx->cap_spill : nullptr;
-                O.spill_count = ctx->d_meta + 4 + (cur ^ 1);
-                O.spill_cap = ctx->cap_spill;
-                const unsigned g2 = (unsigned)std::min<uint64_t>(grid, (pending + gs::kBlockThreads - 1) /
-                                                                          gs::kBlockThreads);
-                HIP_TRY(ctx, gs_launch_traj(q.model, false, ctx->d_params, 1, nullptr, O, g2, ctx->stream));
-            }
-            O.resume = nullptr;
-            O.spill = nullptr;
-            HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-            HIP_TRY(ctx, hipMemcpyAsync(&nev, ctx->d_meta + 2, sizeof nev, hipMemcpyDeviceToHost, ctx->stream));
-            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-            float ms = 0.f;
-            HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-            kernel_ms += ms;
-            if (nev <= ctx->cap_events) break;
-            // The buffer was too small (1D F-p crossings are unbounded): the
-            // trajectories are deterministic, so run the chunk again with room
-            // for every event.  Statistics of the first attempt are discarded.
-            if (attempt == 1) return fail(ctx, GS_ERR_OVERFLOW, "event buffer overflow");
-            st = ensure_events(ctx, nev + nev / 8);
-            if (st != GS_OK) return st;
-            HIP_TRY(ctx, gs_launch_zero(L.stats, gs::ST_COUNT_, ctx->stream));
-        }
-        HIP_TRY(ctx, gs_launch_fold(ctx->d_params, 1, O, fold_lds_bytes(q.n_bins, q.n_weights),
-                                    fold_blocks(nev), ctx->stream));
-        total_events += nev;
-        if (want_recs && nev > 0 && kept.size() < rec_cap) {
-            std::vector<gs::DevEvent> h(nev);
-            HIP_TRY(ctx, hipMemcpyAsync(h.data(), ctx->d_events, nev * sizeof(gs::DevEvent),
-                                        hipMemcpyDeviceToHost, ctx->stream));
-            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-            std::vector<gs_exit_record> rs(nev);
-            for (uint64_t i = 0; i < nev; ++i) rs[i] = to_record(h[i]);
-            std::sort(rs.begin(), rs.end(), record_less);
-            for (uint64_t i = 0; i < nev && kept.size() < rec_cap; ++i) kept.push_back(rs[i]);
-        }
-        if (N == 0) break;
-    }
-    std::vector<uint64_t> host(words_per_sim());
-    HIP_TRY(ctx, hipMemcpyAsync(host.data(), ctx->d_words, host.size() * sizeof(uint64_t),
-                                hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    Layout H = layout(host.data(), 1);
-    H.stats[gs::ST_EVENTS] = total_events;
-    fill_spectrum(*p, H.count, H.digits, H.stats, out);
-    if (want_recs) std::copy(kept.begin(), kept.end(), recs);
-    if (n_recs) *n_recs = total_events;
-    ctx->stats.kernel_ms = kernel_ms;
-    ctx->stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
-    ctx->stats.lane_cycles = H.stats[gs::ST_CYCLES] * (uint64_t)gs::kWave * gs::kCycleSteps;
-    ctx->stats.max_particle_steps = H.stats[gs::ST_MAX_STEPS];
-    ctx->stats.active_cycles = H.stats[gs::ST_STEPS];
-    ctx->stats.grid_blocks = grid;
-    ctx->stats.block_threads = gs::kBlockThreads;
-    if (out->n_acc_overflow) return fail(ctx, GS_ERR_OVERFLOW, "weight accumulator overflow");
-    return GS_OK;
+    PlainBufs B;
+    B.words = ctx->d_words;
+    B.params = ctx->d_params;
+    B.meta = ctx->d_meta;
+    B.own = true;
+    B.events = nullptr;
+    B.cap_events = 0;
+    B.order_counts = nullptr;
+    B.order = nullptr;
+    return run_plain_core(ctx, *p, out, recs, rec_cap, n_recs, B);
 }
 
 gs_status gs_debug_math(gs_ctx *ctx, int32_t fn, const double *x, double *y, uint64_t n) {
@@ -959,6 +1030,14 @@ gs_status gs_set_batch_budget(gs_ctx *ctx, uint64_t max_events) {
     if (!ctx) return GS_ERR_INVALID_ARG;
     if (max_events < (1u << 16)) return fail(ctx, GS_ERR_INVALID_ARG, "budget below 65536 events");
     ctx->batch_event_budget = max_events;
+    return GS_OK;
+}
+
+gs_status gs_set_plain_limits(gs_ctx *ctx, uint64_t max_chunk, uint64_t max_events) {
+    if (!ctx) return GS_ERR_INVALID_ARG;
+    if (max_chunk > kMaxChunk) return fail(ctx, GS_ERR_INVALID_ARG, "max_chunk above 2^27");
+    ctx->plain_chunk = max_chunk ? max_chunk : kMaxChunk;
+    ctx->plain_event_cap = max_events;
     return GS_OK;
 }
 

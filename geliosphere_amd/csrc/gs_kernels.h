@@ -152,6 +152,7 @@ __host__ __device__ inline int lds_words(int nb, int nw) {
 }
 
 constexpr size_t kDigitsPerSim = (size_t)GS_MAX_WEIGHTS * 2 * GS_MAX_BINS * GS_ACC_LIMBS;
+
 }  // namespace gs
 
 hipError_t gs_launch_zero(uint64_t *p, size_t n, hipStream_t s);

@@ -15,6 +15,7 @@ _lib = None
 
 EXPORTS = ("gs_abi_version", "gs_create", "gs_destroy", "gs_last_error", "gs_default_params",
            "gs_validate_params", "gs_run_plain", "gs_run_batch", "gs_set_batch_budget",
+           "gs_set_plain_limits",
            "gs_set_launch", "gs_set_order", "gs_autotune", "gs_last_run_stats", "gs_debug_math", "gs_debug_normals",
            "gs_host_math")
 
@@ -59,6 +60,8 @@ def lib():
         L.gs_run_batch.restype = ctypes.c_int
         L.gs_set_batch_budget.argtypes = [vp, ctypes.c_uint64]
         L.gs_set_batch_budget.restype = ctypes.c_int
+        L.gs_set_plain_limits.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64]
+        L.gs_set_plain_limits.restype = ctypes.c_int
         L.gs_set_launch.argtypes = [vp, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32]
         L.gs_set_launch.restype = ctypes.c_int
         L.gs_set_order.argtypes = [vp, ctypes.c_int32]
@@ -172,6 +175,10 @@ class Engine:
 
     def set_batch_budget(self, max_events):
         self._check(lib().gs_set_batch_budget(self._ctx, max_events), "gs_set_batch_budget")
+
+    def set_plain_limits(self, max_chunk=0, max_events=0):
+        """Particles per plain launch and event records held (0: defaults)."""
+        self._check(lib().gs_set_plain_limits(self._ctx, max_chunk, max_events), "gs_set_plain_limits")
 
     def set_launch(self, model, block_threads=0, blocks_per_cu=0):
         self._check(lib().gs_set_launch(self._ctx, model, block_threads, blocks_per_cu), "gs_set_launch")

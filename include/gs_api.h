@@ -208,6 +208,15 @@ gs_status gs_run_batch(gs_ctx *ctx, const gs_params *sims, uint32_t n_sims,
  * (default 2^29); larger batches run as consecutive sub-batches. */
 gs_status gs_set_batch_budget(gs_ctx *ctx, uint64_t max_events);
 
+/* Memory footprint of gs_run_plain: particles per trajectory launch
+ * (1 .. 2^27; 0 = 2^27) and most exit records held on the device at once
+ * (0 = room for every event of a launch).  A launch whose events exceed the
+ * limit runs again as half as many particles; results do not change.
+ * Replaces the reference's fixed blockSize x threadSize launches and their
+ * record buffers of the same size (CUDAKernel/src/OneDimensionFpGpuModel.cpp:19-20,
+ * OneDimensionFpModel.cu:172-184). */
+gs_status gs_set_plain_limits(gs_ctx *ctx, uint64_t max_chunk, uint64_t max_events);
+
 /* Override the launch shape used for `model` (0 = automatic). */
 gs_status gs_set_launch(gs_ctx *ctx, int32_t model, uint32_t block_threads,
                         uint32_t blocks_per_cu);

@@ -277,3 +277,24 @@ def test_outside_range_events_counted(engine, oracle):
     rc, o, _, _ = oracle.run(p, n_threads=0)
     assert s.n_outside == o.n_outside
     assert s.n_outside + sum(s.count[b] for b in range(10)) == s.n_events
+
+
+def test_overflow_rerun_keeps_earlier_chunks_statistics(engine, oracle):
+    """Plain runs cut into launches of 3000 particles whose event region holds
+    2000 records: a launch overflows (nearly every B-p particle exits), runs
+    again as 1500 particles, and must start again from the statistics of the
+    launches before it.  The result equals the one-launch run bit for bit and
+    the oracle's counts."""
+    p = abi.make_params(abi.MODEL_1D_BP, N=20_000, seed=31, weights=ALL_WEIGHTS)
+    ref, _, nref = engine.run_plain(p)
+    engine.set_plain_limits(3000, 2000)
+    try:
+        cut, _, ncut = engine.run_plain(p)
+        st = engine.last_stats()
+    finally:
+        engine.set_plain_limits(0, 0)
+    assert bytes(cut) == bytes(ref) and ncut == nref
+    assert cut.n_particles == 20_000 and st.max_particle_steps > 0
+    rc, o, _, _ = oracle.run(p, n_threads=0)
+    assert cut.total_steps == o.total_steps and cut.n_events == o.n_events
+    assert [cut.count[b] for b in range(30)] == [o.count[b] for b in range(30)]
