@@ -173,6 +173,14 @@ struct NormalStream {
 // a last-bit perturbation used to measure how far rounding differences grow
 // along a model's trajectories.
 const uint32_t kOracleFlagPerturb = 0x80000000u;
+// Test-only flag bit: every normal is 0, so a step is its deterministic part
+// alone (advection, drifts, energy change), for checks of the step's terms.
+const uint32_t kOracleFlagNoNoise = 0x40000000u;
+
+inline double noise_scale(const gs_params &P) {
+    if (P.flags & kOracleFlagNoNoise) return 0.0;
+    return (P.flags & kOracleFlagPerturb) ? 1.0 + 0x1p-52 : 1.0;
+}
 
 double injection_uniform(uint64_t seed, uint64_t id) {
     const uint32_t ctr[4] = {(uint32_t)id, (uint32_t)(id >> 32), 0u, 1u};
@@ -239,7 +247,7 @@ ParticleEnd particle_bp(const gs_params &P, uint64_t id, const Emitter &emit) {
     const double dt = P.dt, K0 = P.K0;
     const double Tinj = injection_energy(P, id);
     NormalStream ns(P.seed, id);
-    if (P.flags & kOracleFlagPerturb) ns.scale = 1.0 + 0x1p-52;
+    ns.scale = noise_scale(P);
     const uint64_t cap = step_cap(P);
     ParticleEnd end;
     R Tkin = Tinj;
@@ -290,7 +298,7 @@ ParticleEnd particle_fp(const gs_params &P, uint64_t id, const Emitter &emit) {
     const double dt = P.dt, K0 = P.K0;
     const double Tinj = injection_energy(P, id);
     NormalStream ns(P.seed, id);
-    if (P.flags & kOracleFlagPerturb) ns.scale = 1.0 + 0x1p-52;
+    ns.scale = noise_scale(P);
     const uint64_t cap = step_cap(P);
     ParticleEnd end;
     R Tkin = Tinj;
@@ -408,7 +416,7 @@ ParticleEnd particle_sp(const gs_params &P, uint64_t id, const Emitter &emit) {
     const double V = P.V, dt = P.dt;
     const double Tinj = injection_energy(P, id);
     NormalStream ns(P.seed, id);
-    if (P.flags & kOracleFlagPerturb) ns.scale = 1.0 + 0x1p-52;
+    ns.scale = noise_scale(P);
     const uint64_t cap = step_cap(P);
     ParticleEnd end;
     R Tkin = Tinj;
@@ -562,7 +570,7 @@ ParticleEnd particle_ge(const gs_params &P, uint64_t id, const Emitter &emit) {
     const double V = P.V, dt = P.dt;
     const double Tinj = injection_energy(P, id);
     NormalStream ns(P.seed, id);
-    if (P.flags & kOracleFlagPerturb) ns.scale = 1.0 + 0x1p-52;
+    ns.scale = noise_scale(P);
     const uint64_t cap = step_cap(P);
     ParticleEnd end;
     R Tkin = Tinj;

@@ -20,6 +20,9 @@
 /* gs_params.flags bit read by the oracle only: multiply every normal by
  * (1 + 2^-52) to measure the models' sensitivity to last-bit differences. */
 #define ORACLE_FLAG_PERTURB 0x80000000u
+/* gs_params.flags bit read by the oracle only: every normal is 0, so a step
+ * is its deterministic part alone (advection, drifts, energy change). */
+#define ORACLE_FLAG_NO_NOISE 0x40000000u
 
 #ifdef __cplusplus
 extern "C" {
