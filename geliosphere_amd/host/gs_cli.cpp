@@ -6,9 +6,9 @@
 // V in km/s * 6.68458712e-9, N in millions), then runs the model's
 // Algorithm step: simulation, Results pass, removal of log.dat.
 //
-//   gs_cli (-F|-B|-E|-T|-b FILE) [-N millions] [-K K0] [-V V] [-d dt]
-//          [-m month -y year] [-s settings.toml] [-p destination] [-c]
-//          [--evaluation log.dat]
+//   gs_cli (-F|-B|-E|-T|--custom-model NAME|-b FILE) [-N millions] [-K K0]
+//          [-V V] [-d dt] [-m month -y year] [-s settings.toml]
+//          [-p destination] [-c] [--evaluation log.dat] [--cpu-only]
 //          [--r AU] [--theta deg] [--tilt deg] [--polarity +-1]
 //          [--seed N] [--particles N] [--keep-log] [--no-log]
 //          [--solarprop-gpu-compat] [--device N] [--print-params]
@@ -20,12 +20,21 @@
 // next to the executable; they exclude -K and -V, and -m needs -y.
 // --print-params prints the parameter block the model would run with (JSON)
 // and exits without touching a device.
+//
+// The model flags -F, -B, -E, -T, --custom-model and -b exclude each other
+// (ParseParams.cpp:54-59).  --custom-model NAME runs the model the
+// reference's factory returns for NAME (Factory/src/CosmicFactory.cpp:8-31;
+// main.cpp:34-39): one of the four model names, or "batch run" with the
+// default batch file; any other name is an error.  --cpu-only selects the
+// reference's CPU models (ParseParams.cpp:36,136-139), which are not part of
+// this GPU drop-in: it is rejected with a message.
 #include <unistd.h>
 
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "gs_dropin.hpp"
 
@@ -33,9 +42,9 @@ namespace {
 
 void usage() {
     std::fprintf(stderr,
-                 "usage: gs_cli (-F|-B|-E|-T|-b FILE) [-N millions] [-K K0_cm2s] [-V V_kms] [-d dt]\n"
-                 "              [-m month -y year] [-s settings.toml] [-p destination] [-c]\n"
-                 "              [--evaluation LOG]\n"
+                 "usage: gs_cli (-F|-B|-E|-T|--custom-model NAME|-b FILE) [-N millions] [-K K0_cm2s]\n"
+                 "              [-V V_kms] [-d dt] [-m month -y year] [-s settings.toml] [-p destination]\n"
+                 "              [-c] [--evaluation LOG] [--cpu-only]\n"
                  "              [--r AU] [--theta deg]\n"
                  "              [--tilt deg] [--polarity 1|-1] [--seed N] [--particles N]\n"
                  "              [--keep-log] [--no-log] [--solarprop-gpu-compat] [--device N]\n"
@@ -48,7 +57,10 @@ int main(int argc, char **argv) {
     ParamsCarrier *pc = ParamsCarrier::instance();
     int model = -1;
     std::string batch;
+    std::string custom_model;
     bool print_params = false;
+    bool cpu_only = false;
+    std::vector<std::string> model_flags;   // -F -B -E -T --custom-model -b, as given
     int month = -1, year = -1;
     std::string settings;
     bool settings_set = false;
@@ -68,11 +80,17 @@ int main(int argc, char **argv) {
             }
             return argv[++i];
         };
-        if (a == "-F" || a == "--forward") model = GS_MODEL_1D_FP;
-        else if (a == "-B" || a == "--backward") model = GS_MODEL_1D_BP;
-        else if (a == "-E" || a == "--solarprop-like-model") model = GS_MODEL_2D_SOLARPROP;
-        else if (a == "-T" || a == "--geliosphere-2d-model") model = GS_MODEL_2D_GELIOSPHERE;
-        else if (a == "-b" || a == "--batchrun") batch = val();
+        if (a == "-F" || a == "--forward") { model = GS_MODEL_1D_FP; model_flags.push_back("--forward"); }
+        else if (a == "-B" || a == "--backward") { model = GS_MODEL_1D_BP; model_flags.push_back("--backward"); }
+        else if (a == "-E" || a == "--solarprop-like-model") {
+            model = GS_MODEL_2D_SOLARPROP;
+            model_flags.push_back("--solarprop-like-model");
+        } else if (a == "-T" || a == "--geliosphere-2d-model") {
+            model = GS_MODEL_2D_GELIOSPHERE;
+            model_flags.push_back("--geliosphere-2d-model");
+        } else if (a == "--custom-model") { custom_model = val(); model_flags.push_back("--custom-model"); }
+        else if (a == "-b" || a == "--batchrun") { batch = val(); model_flags.push_back("--batchrun"); }
+        else if (a == "--cpu-only") cpu_only = true;
         else if (a == "-c" || a == "--csv") pc->putInt("csv", 1);
         else if (a == "-N" || a == "--number-of-test-particles") {
             const int n = std::atoi(val());
@@ -122,7 +140,21 @@ int main(int argc, char **argv) {
             return 2;
         }
     }
-    // option relations of the reference CLI (ParseParams.cpp:49-62)
+    // option relations of the reference CLI (ParseParams.cpp:51-62)
+    if (model_flags.size() > 1) {
+        std::fprintf(stderr, "%s excludes %s\n", model_flags[1].c_str(), model_flags[0].c_str());
+        return 2;
+    }
+    if (cpu_only) {
+        std::fprintf(stderr, "[error] --cpu-only selects the reference's CPU models (CpuImplementations/), which "
+                             "this GPU drop-in does not provide; build the reference with GPU_ENABLED=0 for them.\n");
+        return 2;
+    }
+    if (!custom_model.empty()) {
+        // the reference's factory: the four model names and "batch run"
+        if (custom_model == "batch run") batch = "../geliosphere_batch_input_paramaters.csv";
+        else model = gsdrop::model_from_name(custom_model);
+    }
     const bool date = month != -1 || year != -1;
     if ((k_set || v_set) && date) { std::fprintf(stderr, "-K/-V exclude -m/-y\n"); return 2; }
     if (month != -1 && year == -1) { std::fprintf(stderr, "-m requires -y\n"); return 2; }
@@ -138,8 +170,11 @@ int main(int argc, char **argv) {
         std::fprintf(stderr, settings_set ? "[warning] No settings file exists on entered path.\n"
                                           : "[warning] No settings file exists on default path.\n");
     }
+    if (!custom_model.empty() && batch.empty() && model < 0) {
+        std::fprintf(stderr, "[error] Selected custom model is not defined in factory.\n");
+        return 255;
+    }
     if (!batch.empty()) {
-        if (model >= 0) { std::fprintf(stderr, "-b excludes the model flags\n"); return 2; }
         if (k_set || v_set || other_set || date) {
             std::fprintf(stderr, "-b excludes -d, -N, -K, -V, -m and -y\n");
             return 2;

@@ -175,6 +175,53 @@ def test_batch_csv_rows(tmp_path):
     assert rows[1]["K0"] == -1 and rows[1]["N"] == 2 and rows[1]["name"] == "run B" and rows[1]["year"] == 2010
 
 
+def test_model_flags_exclude_each_other():
+    """-F, -B, -E, -T, --custom-model and -b exclude each other
+    (Input/src/ParseParams.cpp:54-59); the run does not start."""
+    for a, b in [("-B", "-F"), ("-E", "-T"), ("-F", "-T"), ("-B", "--custom-model")]:
+        args = [a, b] if b != "--custom-model" else [a, b, "1D Bp"]
+        r = cli(*args, "--print-params", check=False)
+        assert r.returncode != 0 and "excludes" in r.stderr and r.stdout == ""
+    r = cli("-b", "x.csv", "-B", "--print-params", check=False)
+    assert r.returncode != 0 and "excludes" in r.stderr
+
+
+def test_custom_model_resolves_through_the_factory():
+    """--custom-model NAME runs what the reference's factory returns for NAME
+    (Factory/src/CosmicFactory.cpp:8-31): a model name selects that model;
+    an unknown name is an error (main.cpp:34-39)."""
+    assert params("--custom-model", "2D Geliosphere", "--seed", "1")["model"] == 3
+    assert params("--custom-model", "1D Fp", "--seed", "1")["model"] == 0
+    r = cli("--custom-model", "My model", "--print-params", check=False)
+    assert r.returncode != 0 and "not defined in factory" in r.stderr
+
+
+def test_cpu_only_is_rejected_with_a_message():
+    """--cpu-only selects the reference's CPU models (ParseParams.cpp:36,
+    136-139), which the GPU drop-in does not provide."""
+    r = cli("-B", "--cpu-only", "--print-params", check=False)
+    assert r.returncode != 0 and "--cpu-only" in r.stderr and r.stdout == ""
+
+
+def test_batch_duplicate_names_reject_the_batch(tmp_path):
+    """Non-blank names must be unique (BatchRun.cpp:344-365); blank and empty
+    names may repeat.  A row with model "batch run" is skipped."""
+    f = tmp_path / "dup.csv"
+    f.write_text("year,month,K0,V,dt,N,r,theta,pathToCustomSettingsFile,name,model\n"
+                 "-1,-1,5e22,400,500,1,1.0,90,,same,1D Bp\n"
+                 "-1,-1,5e22,400,500,1,1.0,90,,,1D Bp\n"
+                 "-1,-1,5e22,400,500,1,1.0,90,,same,1D Fp\n")
+    r = cli("-b", str(f), "--print-params", check=False)
+    assert r.returncode != 0 and "duplicated values" in r.stderr and r.stdout == ""
+    f.write_text("year,month,K0,V,dt,N,r,theta,pathToCustomSettingsFile,name,model\n"
+                 "-1,-1,5e22,400,500,1,1.0,90,,a,1D Bp\n"
+                 "-1,-1,5e22,400,500,1,1.0,90,,  ,1D Bp\n"
+                 "-1,-1,5e22,400,500,1,1.0,90,,  ,1D Bp\n"
+                 "-1,-1,5e22,400,500,1,1.0,90,,,1D Fp\n"
+                 "-1,-1,5e22,400,500,1,1.0,90,,,1D Fp\n")
+    assert len(cli("-b", str(f), "--print-params").stdout.splitlines()) == 5
+
+
 # ---------------------------------------------------------------- Results passes
 
 def p_si(T):
