@@ -382,3 +382,35 @@ def test_ulysses_fixture_and_settings():
     assert all(1.0 < r["r"] < 6.0 and 0.0 < r["theta"] < 180.0 for r in rows)
     assert all(r["ket_p190"] > 0 and r["ket_p116"] > 0 for r in rows)
     assert [r["cr"] for r in rows] == list(range(rows[0]["cr"], rows[0]["cr"] + 67))
+
+
+REF_INPUT = "/root/reference/Input"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_INPUT), reason="the reference tree is not present on this machine")
+def test_dropin_builds_against_the_reference_params_carrier(tmp_path):
+    """The drop-in host code compiles and links against the reference's own
+    ParamsCarrier (Input/include/ParamsCarrier.hpp and Input/src/
+    ParamsCarrier.cpp, read where they lie; compat/ is not on the include
+    path), and gs_cli built that way maps flags, settings and batch rows to
+    the same parameter blocks as the standalone build."""
+    host = os.path.join(ROOT, "geliosphere_amd", "host")
+    exe = tmp_path / "gs_cli_ref"
+    srcs = [os.path.join(host, f) for f in ("gs_cli.cpp", "gs_dropin.cpp", "gs_results.cpp", "gs_batch.cpp",
+                                            "gs_tables.cpp", "gs_settings.cpp")]
+    cmd = ["g++", "-std=c++17", "-O1", "-I", os.path.join(REF_INPUT, "include"), "-I", host, "-o", str(exe), *srcs,
+           os.path.join(REF_INPUT, "src", "ParamsCarrier.cpp"), "-L", os.path.join(ROOT, "geliosphere_amd"),
+           "-lgeliosphere_amd", "-Wl,-rpath," + os.path.join(ROOT, "geliosphere_amd")]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    st = settings_file(tmp_path, r_injection=5.0)
+    b = tmp_path / "b.csv"
+    b.write_text("year,month,K0,V,dt,N,r,theta,pathToCustomSettingsFile,name,model\n"
+                 "-1,-1,5e22,400,500,1,1.0,90,,runA,1D Bp\n-1,-1,,,,2,30,45,,,2D Geliosphere\n")
+    for args in (["-B", "--seed", "5"], ["-F", "-K", "3e22", "-V", "600", "-d", "100", "--seed", "1"],
+                 ["-T", "-K", "1e22", "-V", "500", "--tilt", "40", "--polarity", "-1", "--seed", "2"],
+                 ["-E", "-s", str(st), "--seed", "3"], ["--custom-model", "2D SolarProp-like", "--seed", "4"],
+                 ["-b", str(b)]):
+        ref = subprocess.run([str(exe), *args, "--print-params"], capture_output=True, text=True, timeout=60)
+        own = cli(*args, "--print-params")
+        assert ref.returncode == 0 and ref.stdout == own.stdout, args
