@@ -71,6 +71,10 @@ struct gs_ctx {
     uint64_t cap_spill = 0;
     uint32_t block_override[4] = {0, 0, 0, 0};
     uint32_t bpcu_override[4] = {0, 0, 0, 0};
+    // the device's fixed scratch array (gs_run_plain_noalloc), looked up once
+    // per device when the context is created
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
     gs_run_stats stats{};
     std::string err;
 };
@@ -387,6 +391,18 @@ double est_cost(const gs_params &p) {
 }
 
 const size_t kMaxBatchSims = 65535;   // fold grid.y limit
+
+// Layout of the fixed scratch (64-bit words) of gs_run_plain_noalloc: the
+// histogram words of one simulation, its parameters, the run counters, the
+// energy-order counters and permutation, the event region.
+constexpr size_t kScratchParamsAt = (kWordsPerSim + 31) / 32 * 32;   // 256-byte aligned
+constexpr size_t kScratchMetaAt = kScratchParamsAt + (sizeof(DevParams) + 255) / 256 * 32;
+static_assert(kScratchMetaAt + 64 <= gs::kScratchHeadWords, "scratch head too small");
+constexpr size_t kScratchCountsAt = gs::kScratchHeadWords;
+constexpr size_t kScratchPermAt = kScratchCountsAt + gs::kScratchCountWords;
+constexpr size_t kScratchEventsAt = kScratchPermAt + gs::kScratchPermWords;
+static_assert(kScratchEventsAt % 8 == 0, "event region 64-byte aligned");
+static_assert(kScratchEventsAt + gs::kScratchEvents * 8 == gs::kScratchWords, "scratch layout");
 
 gs_status run_sub_batch(gs_ctx *ctx, const gs_params *sims, const std::vector<uint32_t> &part,
                         gs_spectrum *outs, std::vector<uint32_t> &redo, gs_run_stats &acc);
@@ -817,6 +833,8 @@ gs_status gs_create(gs_ctx **out, int device) {
         if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->join[i], hipEventDisableTiming);
     }
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming);
+    // the device's scratch array, with the device current (looked up once per device)
+    if (e == hipSuccess) e = gs_scratch_address(device, &ctx->scratch, &ctx->scratch_bytes);
     if (e != hipSuccess) {
         gs_destroy(ctx);
         return GS_ERR_HIP;
@@ -962,6 +980,30 @@ gs_status gs_run_plain(gs_ctx *ctx, const gs_params *p, gs_spectrum *out, gs_exi
     B.order_counts = nullptr;
     B.order = nullptr;
     return run_plain_core(ctx, *p, out, recs, rec_cap, n_recs, B);
+}
+
+gs_status gs_run_plain_noalloc(gs_ctx *ctx, const gs_params *p, gs_spectrum *out) {
+    if (!ctx) return GS_ERR_INVALID_ARG;
+    if (!out) return fail(ctx, GS_ERR_INVALID_ARG, "out is NULL");
+    if (p && (p->n_bins > GS_MAX_BINS || p->n_weights > GS_MAX_WEIGHTS))
+        return fail(ctx, GS_ERR_INVALID_ARG, "histogram of %u bins x %u weights exceeds the scratch maximum of %u x %u",
+                    (unsigned)p->n_bins, (unsigned)p->n_weights, (unsigned)GS_MAX_BINS, (unsigned)GS_MAX_WEIGHTS);
+    std::string msg;
+    if (!check_params(p, msg)) return fail(ctx, GS_ERR_INVALID_ARG, "invalid params: %s", msg.c_str());
+    if (!ctx->scratch || ctx->scratch_bytes < gs::kScratchWords * sizeof(uint64_t))
+        return fail(ctx, GS_ERR_HIP, "no scratch array on device %d", ctx->device);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    uint64_t *base = (uint64_t *)ctx->scratch;   // the device address, never the symbol
+    PlainBufs B;
+    B.words = base;
+    B.params = (DevParams *)(base + kScratchParamsAt);
+    B.meta = base + kScratchMetaAt;
+    B.own = false;
+    B.events = (gs::DevEvent *)(base + kScratchEventsAt);
+    B.cap_events = gs::kScratchEvents;
+    B.order_counts = (uint32_t *)(base + kScratchCountsAt);
+    B.order = (uint32_t *)(base + kScratchPermAt);
+    return run_plain_core(ctx, *p, out, nullptr, 0, nullptr, B);
 }
 
 gs_status gs_debug_math(gs_ctx *ctx, int32_t fn, const double *x, double *y, uint64_t n) {

@@ -153,7 +153,23 @@ __host__ __device__ inline int lds_words(int nb, int nw) {
 
 constexpr size_t kDigitsPerSim = (size_t)GS_MAX_WEIGHTS * 2 * GS_MAX_BINS * GS_ACC_LIMBS;
 
+// Fixed device scratch of the no-allocation plain run (gs_run_plain_noalloc),
+// in 64-bit words: one simulation's histogram words, its parameters and run
+// counters (kScratchHeadWords), the energy-order counters and permutation of
+// up to kScratchEvents particles, and an event region of kScratchEvents
+// 64-byte records.  34.2 MiB per device.
+constexpr uint64_t kScratchEvents = 1ull << 19;
+constexpr size_t kScratchHeadWords = 8704;
+constexpr size_t kScratchCountWords = kOrderMaxBuckets;   // 2 x kOrderMaxBuckets uint32
+constexpr size_t kScratchPermWords = kScratchEvents / 2;  // kScratchEvents uint32
+constexpr size_t kScratchWords =
+    kScratchHeadWords + kScratchCountWords + kScratchPermWords + kScratchEvents * (sizeof(DevEvent) / 8);
 }  // namespace gs
+
+// Device address and size of `device`'s scratch array.  The first call for a
+// device must be made with that device current; the address is then kept
+// per device.
+hipError_t gs_scratch_address(int device, void **ptr, size_t *bytes);
 
 hipError_t gs_launch_zero(uint64_t *p, size_t n, hipStream_t s);
 hipError_t gs_launch_debug(int fn, const double *x, double *y, uint64_t n, uint64_t seed,

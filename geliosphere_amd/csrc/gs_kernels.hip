@@ -12,6 +12,7 @@
 // (CUDAKernel/src/OneDimensionBpModel.cu:171-184); here no lane waits for
 // another lane's particle except at the very end of the run.
 #include <algorithm>
+#include <mutex>
 
 #include "gs_device.hpp"
 #include "gs_kernels.h"
@@ -704,6 +705,39 @@ __global__ void zero_kernel(uint64_t *p, size_t n) {
 }
 
 }  // namespace gs
+
+// ---------------------------------------------------------------- scratch
+// The no-allocation plain run's working memory: a fixed array in every
+// device's global memory, held until the process ends.  Host code must not
+// use its name as an address: the symbol's device address is looked up once
+// per device (hipGetSymbolAddress, with that device current), and that
+// address is what kernels and copies receive.
+static __device__ __align__(256) uint64_t scratch[gs::kScratchWords];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t gs_scratch_address(int device, void **ptr, size_t *bytes) {
+    if (device < 0 || device >= kMaxDevices || !ptr || !bytes) return hipErrorInvalidValue;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device) return hipErrorInvalidDevice;
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[device] = p;
+    }
+    *ptr = g_scratch_ptr[device];
+    *bytes = sizeof(scratch);
+    return hipSuccess;
+}
 
 // ---------------------------------------------------------------- launchers
 using namespace gs;

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libgeliosphere_amd.so")
 _lib = None
 
 EXPORTS = ("gs_abi_version", "gs_create", "gs_destroy", "gs_last_error", "gs_default_params",
-           "gs_validate_params", "gs_run_plain", "gs_run_batch", "gs_set_batch_budget",
+           "gs_validate_params", "gs_run_plain", "gs_run_plain_noalloc", "gs_run_batch", "gs_set_batch_budget",
            "gs_set_plain_limits",
            "gs_set_launch", "gs_set_order", "gs_autotune", "gs_last_run_stats", "gs_debug_math", "gs_debug_normals",
            "gs_host_math")
@@ -56,6 +56,8 @@ def lib():
         L.gs_run_plain.argtypes = [vp, P(abi.GsParams), P(abi.GsSpectrum), P(abi.GsExitRecord),
                                    ctypes.c_uint64, P(ctypes.c_uint64)]
         L.gs_run_plain.restype = ctypes.c_int
+        L.gs_run_plain_noalloc.argtypes = [vp, P(abi.GsParams), P(abi.GsSpectrum)]
+        L.gs_run_plain_noalloc.restype = ctypes.c_int
         L.gs_run_batch.argtypes = [vp, P(abi.GsParams), ctypes.c_uint32, P(abi.GsSpectrum)]
         L.gs_run_batch.restype = ctypes.c_int
         L.gs_set_batch_budget.argtypes = [vp, ctypes.c_uint64]
@@ -152,6 +154,14 @@ class Engine:
         rc = lib().gs_run_plain(self._ctx, ctypes.byref(p), ctypes.byref(s), None, 0, ctypes.byref(n))
         self._check(rc, "gs_run_plain")
         return s, None, n.value
+
+    def run_plain_noalloc(self, p):
+        """gs_run_plain_noalloc: the histogram of one run, computed in the
+        device's fixed scratch without any allocation (no exit records)."""
+        s = abi.GsSpectrum()
+        rc = lib().gs_run_plain_noalloc(self._ctx, ctypes.byref(p), ctypes.byref(s))
+        self._check(rc, "gs_run_plain_noalloc")
+        return s
 
     def run_plain_status(self, p):
         """Status code of a run (no raise), for error-path tests."""

@@ -195,6 +195,23 @@ gs_status gs_run_plain(gs_ctx *ctx, const gs_params *p, gs_spectrum *out,
                        gs_exit_record *recs, uint64_t rec_cap,
                        uint64_t *n_recs);
 
+/* gs_run_plain without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under stream
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It makes no
+ * hipMalloc, hipFree or other allocating call.  Its working memory is a fixed
+ * scratch array of 34.2 MiB in each device's global memory, held until the
+ * process ends: one simulation's histogram, parameters and counters, the
+ * energy order of 2^19 particles and an event region of 2^19 records.  A run
+ * goes through it in launches of at most 2^19 particles (1D F-p: 2^23, halved
+ * while a launch's events overflow the region).  Histograms up to
+ * GS_MAX_BINS x GS_MAX_WEIGHTS fit; larger ones, and a single particle with
+ * more than 2^19 events, are rejected (GS_ERR_INVALID_ARG, GS_ERR_OVERFLOW).
+ * The result equals gs_run_plain(ctx, p, out, NULL, 0, NULL) bit for bit; no
+ * exit records are returned.  Every call on a device shares its scratch,
+ * which keeps the last call's contents (each call clears what it uses):
+ * calls that overlap on one device, from any contexts or threads, race. */
+gs_status gs_run_plain_noalloc(gs_ctx *ctx, const gs_params *p, gs_spectrum *out);
+
 /* Run n_sims independent simulations (any mix of models) together: one
  * persistent launch per model over the concatenated particles of its
  * simulations, the model groups concurrently, then one histogram pass; the
