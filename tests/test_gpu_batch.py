@@ -17,7 +17,7 @@ from geliosphere_amd import abi
 
 pytestmark = pytest.mark.gpu
 
-ALL_WEIGHTS = (abi.W_POWERLAW, abi.W_YAMADA, abi.W_USOSKIN, abi.W_BURGER)
+ALL_WEIGHTS = (abi.W_POWERLAW, abi.W_YAMADA, abi.W_WEBER, abi.W_USOSKIN, abi.W_BURGER)
 
 
 def say(*a):
@@ -124,7 +124,7 @@ def test_batch_errors(engine):
 
 
 def test_autotune_picks_a_valid_shape_and_keeps_results(engine):
-    p = abi.make_params(abi.MODEL_1D_BP, N=200_000, seed=31, weights=ALL_WEIGHTS)
+    p = abi.make_params(abi.MODEL_1D_BP, N=200_000, seed=31, weights=ALL_WEIGHTS[:4])
     ref, _, _ = engine.run_plain(p)
     try:
         bpcu, rate = engine.autotune(p)
@@ -134,3 +134,52 @@ def test_autotune_picks_a_valid_shape_and_keeps_results(engine):
     finally:
         engine.set_launch(abi.MODEL_1D_BP, 0, 0)
     assert bytes(ref) == bytes(again)
+
+
+def test_config4_sampled_batch_matches_oracle(engine, oracle):
+    """Sixteen simulations drawn by the batch benchmark's sampler (four of
+    each model; K0 log-uniform 1e22-1e23 cm^2/s, V 300-800 km/s or tilt
+    5-75 deg, r 1-80 AU, theta 10-170 deg, dt in {50, 100, 500} s, A +-1),
+    at particle counts the oracle finishes in seconds, run as one batch and
+    compared with the oracle at the same seed: 1D simulations bin for bin
+    (counts and steps identical, sums to 1e-9); 2D simulations, whose
+    trajectories amplify last-bit differences of sin/cos/atan, with
+    identical per-bin counts (the bin is the injection energy), steps within
+    2 % and per-bin mean weights within 3 SE.  The reference runs such rows
+    one after the other (Algorithm/src/BatchRun.cpp:26-79)."""
+    from geliosphere_amd.batch import sample_one
+    from tests.parity import compare_spectra
+    rng = random.Random(4040)
+    n_small = {abi.MODEL_1D_FP: 200_000, abi.MODEL_1D_BP: 1_500, abi.MODEL_2D_SOLARPROP: 150,
+               abi.MODEL_2D_GELIOSPHERE: 150}
+    w1d = (abi.W_POWERLAW, abi.W_YAMADA, abi.W_WEBER, abi.W_USOSKIN)
+    w2d = (abi.W_POWERLAW, abi.W_WEBER, abi.W_USOSKIN, abi.W_BURGER)
+    sims = []
+    for i in range(16):
+        model = (abi.MODEL_1D_FP, abi.MODEL_1D_BP, abi.MODEL_2D_SOLARPROP, abi.MODEL_2D_GELIOSPHERE)[i % 4]
+        p = sample_one(rng, 9000 + i, model=model)
+        p.n_particles = n_small[model]
+        w = w1d if model < 2 else w2d
+        p.n_weights = len(w)
+        for k, kind in enumerate(w):
+            p.weights[k] = kind
+        sims.append(p)
+    outs = engine.run_batch(sims)
+    for i, (p, g) in enumerate(zip(sims, outs)):
+        rc, o, _, _ = oracle.run(p, n_threads=0)
+        assert rc == 0
+        say("sim %2d %-18s r %5.1f dt %3.0f  steps gpu %d oracle %d  events %d %d"
+            % (i, abi.MODEL_NAMES[p.model], p.r_inj, p.dt, g.total_steps, o.total_steps, g.n_events, o.n_events))
+        assert g.n_particles == o.n_particles == p.n_particles
+        assert [g.count[b] for b in range(30)] == [o.count[b] for b in range(30)]
+        assert g.n_events == o.n_events and g.n_outside == o.n_outside
+        if p.model < 2:
+            assert g.total_steps == o.total_steps
+            for k in range(4):
+                for b in range(30):
+                    assert g.sum_w[k][b] == pytest.approx(o.sum_w[k][b], rel=1e-9, abs=0.0)
+        else:
+            assert abs(g.total_steps - o.total_steps) <= 0.02 * o.total_steps
+            for k in range(4):
+                zs, chi2, dof, pval = compare_spectra(g, o, k=k, min_count=5)
+                assert all(abs(z) < 3.0 for z in zs), (i, k, zs)

@@ -27,7 +27,7 @@ MODELS = [abi.MODEL_1D_BP, abi.MODEL_1D_FP, abi.MODEL_2D_SOLARPROP, abi.MODEL_2D
 SAME_SEED_N = {abi.MODEL_1D_BP: 4000, abi.MODEL_1D_FP: 400000,
                abi.MODEL_2D_SOLARPROP: 1500, abi.MODEL_2D_GELIOSPHERE: 1200}
 ORACLE_FLAG_PERTURB = 0x80000000   # oracle/oracle.h
-ALL_WEIGHTS = (abi.W_POWERLAW, abi.W_YAMADA, abi.W_USOSKIN, abi.W_BURGER)
+ALL_WEIGHTS = (abi.W_POWERLAW, abi.W_YAMADA, abi.W_WEBER, abi.W_BURGER)
 
 
 def say(*a):
@@ -120,7 +120,8 @@ def test_same_seed_particle_parity(engine, oracle, model):
 # reference GPU's SolarProp drift variant, particle by particle.
 SAME_SEED_CASES = [
     ("1D B-p grid injection, shard offset, r=5 AU, dt=100 s",
-     dict(model=abi.MODEL_1D_BP, N=3000, injection=abi.INJ_GRID, grid_n=997, first=1_000_003, r_inj=5.0, dt=100.0)),
+     dict(model=abi.MODEL_1D_BP, N=3000, injection=abi.INJ_GRID, grid_n=997, first=1_000_003, r_inj=5.0, dt=100.0,
+          weights=(abi.W_USOSKIN, abi.W_WEBER, abi.W_POWERLAW, abi.W_YAMADA))),
     ("1D F-p grid injection, V=600 km/s",
      dict(model=abi.MODEL_1D_FP, N=1_000_000, injection=abi.INJ_GRID, grid_n=5000, V_kms=600.0)),
     ("SolarProp bins, reference GPU drift, A<0, tilt 75, theta 60, r=3 AU",
@@ -131,6 +132,9 @@ SAME_SEED_CASES = [
           tilt_deg=20.0, theta_deg=1.0)),
     ("Geliosphere log-uniform, r=30 AU, theta 150, shard offset",
      dict(model=abi.MODEL_2D_GELIOSPHERE, N=1000, r_inj=30.0, theta_deg=150.0, tilt_deg=60.0, first=777_777)),
+    ("Geliosphere polar cap, theta 1 deg, delta0 = 0.003 sin(1 deg) (the polar terms are exact derivatives)",
+     dict(model=abi.MODEL_2D_GELIOSPHERE, N=1000, theta_deg=1.0, tilt_deg=30.0,
+          delta0=0.003 * math.sin(math.radians(1.0)))),
 ]
 
 
@@ -138,7 +142,8 @@ SAME_SEED_CASES = [
 def test_same_seed_particle_parity_configs(engine, oracle, label, kw):
     kw = dict(kw)
     model = kw.pop("model")
-    p = abi.make_params(model, seed=424242, weights=ALL_WEIGHTS, **kw)
+    weights = kw.pop("weights", ALL_WEIGHTS)
+    p = abi.make_params(model, seed=424242, weights=weights, **kw)
     same_seed_check(engine, oracle, p, label)
 
 
