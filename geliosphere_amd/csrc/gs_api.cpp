@@ -404,8 +404,16 @@ constexpr size_t kScratchEventsAt = kScratchPermAt + gs::kScratchPermWords;
 static_assert(kScratchEventsAt % 8 == 0, "event region 64-byte aligned");
 static_assert(kScratchEventsAt + gs::kScratchEvents * 8 == gs::kScratchWords, "scratch layout");
 
+// Per-simulation exit records wanted from a batch (all three NULL: none).
+struct BatchRecs {
+    gs_exit_record *const *recs;
+    const uint64_t *caps;
+    uint64_t *n_recs;
+    bool want(uint32_t i) const { return recs && recs[i] && caps && caps[i] > 0; }
+};
+
 gs_status run_sub_batch(gs_ctx *ctx, const gs_params *sims, const std::vector<uint32_t> &part,
-                        gs_spectrum *outs, std::vector<uint32_t> &redo, gs_run_stats &acc);
+                        gs_spectrum *outs, std::vector<uint32_t> &redo, gs_run_stats &acc, const BatchRecs &R);
 
 }  // namespace
 
@@ -418,7 +426,7 @@ namespace {
 // The launch sequence is captured into a hipGraph once per layout and
 // replayed while the layout (counts, grids, buffers) stays the same.
 gs_status run_sub_batch(gs_ctx *ctx, const gs_params *sims, const std::vector<uint32_t> &part,
-                        gs_spectrum *outs, std::vector<uint32_t> &redo, gs_run_stats &acc) {
+                        gs_spectrum *outs, std::vector<uint32_t> &redo, gs_run_stats &acc, const BatchRecs &R) {
     const size_t m = part.size();
     // ---- order
     std::vector<uint32_t> order(part);
@@ -608,6 +616,18 @@ gs_status run_sub_batch(gs_ctx *ctx, const gs_params *sims, const std::vector<ui
         uint64_t *stats = H.stats + j * gs::ST_COUNT_;
         stats[gs::ST_EVENTS] = nev;
         fill_spectrum(sims[i], H.count + j * GS_MAX_BINS, H.digits + j * gs::kDigitsPerSim, stats, &outs[i]);
+        if (R.n_recs) R.n_recs[i] = nev;
+        if (R.want(i) && nev > 0) {
+            // the simulation's event region, in gs_run_plain's record order
+            std::vector<gs::DevEvent> h(nev);
+            HIP_TRY(ctx, hipMemcpyAsync(h.data(), ctx->d_events + hmeta[j], nev * sizeof(gs::DevEvent),
+                                        hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            std::vector<gs_exit_record> rs(nev);
+            for (uint64_t e = 0; e < nev; ++e) rs[e] = to_record(h[e]);
+            std::sort(rs.begin(), rs.end(), record_less);
+            std::copy(rs.begin(), rs.begin() + (ptrdiff_t)std::min<uint64_t>(nev, R.caps[i]), R.recs[i]);
+        }
         acc.active_cycles += stats[gs::ST_STEPS];
     }
     for (size_t l = 0; l < nl; ++l) {
@@ -1083,7 +1103,8 @@ gs_status gs_set_plain_limits(gs_ctx *ctx, uint64_t max_chunk, uint64_t max_even
     return GS_OK;
 }
 
-gs_status gs_run_batch(gs_ctx *ctx, const gs_params *sims, uint32_t n_sims, gs_spectrum *outs) {
+gs_status gs_run_batch_records(gs_ctx *ctx, const gs_params *sims, uint32_t n_sims, gs_spectrum *outs,
+                               gs_exit_record *const *recs, const uint64_t *rec_caps, uint64_t *n_recs) {
     if (!ctx) return GS_ERR_INVALID_ARG;
     if ((!sims || !outs) && n_sims) return fail(ctx, GS_ERR_INVALID_ARG, "NULL argument");
     for (uint32_t i = 0; i < n_sims; ++i) {
@@ -1091,6 +1112,8 @@ gs_status gs_run_batch(gs_ctx *ctx, const gs_params *sims, uint32_t n_sims, gs_s
         if (!check_params(&sims[i], msg))
             return fail(ctx, GS_ERR_INVALID_ARG, "invalid params of simulation %u: %s", i, msg.c_str());
     }
+    if (recs && !rec_caps) return fail(ctx, GS_ERR_INVALID_ARG, "records without capacities");
+    const BatchRecs R{recs, rec_caps, n_recs};
     const auto t_start = std::chrono::steady_clock::now();
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     gs_run_stats acc{};
@@ -1100,7 +1123,8 @@ gs_status gs_run_batch(gs_ctx *ctx, const gs_params *sims, uint32_t n_sims, gs_s
     for (uint32_t i = 0; i < n_sims; ++i) {
         const gs_params &p = sims[i];
         if (p.n_particles > kMaxChunk || event_room(p.model, p.n_particles) > ctx->batch_event_budget / 2) {
-            gs_status st = gs_run_plain(ctx, &p, &outs[i], nullptr, 0, nullptr);
+            gs_status st = gs_run_plain(ctx, &p, &outs[i], R.want(i) ? recs[i] : nullptr, R.want(i) ? rec_caps[i] : 0,
+                                        n_recs ? &n_recs[i] : nullptr);
             if (st != GS_OK) return st;
             acc.kernel_ms += ctx->stats.kernel_ms;
             acc.lane_cycles += ctx->stats.lane_cycles;
@@ -1126,10 +1150,11 @@ gs_status gs_run_batch(gs_ctx *ctx, const gs_params *sims, uint32_t n_sims, gs_s
         }
         std::vector<uint32_t> part(small.begin() + i0, small.begin() + i1);
         std::vector<uint32_t> redo;
-        gs_status st = run_sub_batch(ctx, sims, part, outs, redo, acc);
+        gs_status st = run_sub_batch(ctx, sims, part, outs, redo, acc, R);
         if (st != GS_OK) return st;
         for (uint32_t i : redo) {   // event region overflow (1D F-p): run alone
-            st = gs_run_plain(ctx, &sims[i], &outs[i], nullptr, 0, nullptr);
+            st = gs_run_plain(ctx, &sims[i], &outs[i], R.want(i) ? recs[i] : nullptr, R.want(i) ? rec_caps[i] : 0,
+                              n_recs ? &n_recs[i] : nullptr);
             if (st != GS_OK) return st;
             acc.kernel_ms += ctx->stats.kernel_ms;
             acc.lane_cycles += ctx->stats.lane_cycles;
@@ -1142,6 +1167,10 @@ gs_status gs_run_batch(gs_ctx *ctx, const gs_params *sims, uint32_t n_sims, gs_s
     for (uint32_t i = 0; i < n_sims; ++i)
         if (outs[i].n_acc_overflow) return fail(ctx, GS_ERR_OVERFLOW, "weight accumulator overflow (simulation %u)", i);
     return GS_OK;
+}
+
+gs_status gs_run_batch(gs_ctx *ctx, const gs_params *sims, uint32_t n_sims, gs_spectrum *outs) {
+    return gs_run_batch_records(ctx, sims, n_sims, outs, nullptr, nullptr, nullptr);
 }
 
 }  // extern "C"

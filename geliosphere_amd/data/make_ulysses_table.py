@@ -7,7 +7,7 @@ the Ulysses KET P116/P190 proton fluxes (KET_P116_P190_1994-1998.csv)
 averaged over the rotation (the 27 days from its start).  Data only; run
 from the repository root:
 
-    python3 tests/golden/make_ulysses_fixture.py /root/reference
+    python3 geliosphere_amd/data/make_ulysses_table.py /root/reference
 """
 import csv
 import datetime

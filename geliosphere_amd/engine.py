@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libgeliosphere_amd.so")
 _lib = None
 
 EXPORTS = ("gs_abi_version", "gs_create", "gs_destroy", "gs_last_error", "gs_default_params",
-           "gs_validate_params", "gs_run_plain", "gs_run_plain_noalloc", "gs_run_batch", "gs_set_batch_budget",
+           "gs_validate_params", "gs_run_plain", "gs_run_plain_noalloc", "gs_run_batch", "gs_run_batch_records", "gs_set_batch_budget",
            "gs_set_plain_limits",
            "gs_set_launch", "gs_set_order", "gs_autotune", "gs_last_run_stats", "gs_debug_math", "gs_debug_normals",
            "gs_host_math")
@@ -60,6 +60,9 @@ def lib():
         L.gs_run_plain_noalloc.restype = ctypes.c_int
         L.gs_run_batch.argtypes = [vp, P(abi.GsParams), ctypes.c_uint32, P(abi.GsSpectrum)]
         L.gs_run_batch.restype = ctypes.c_int
+        L.gs_run_batch_records.argtypes = [vp, P(abi.GsParams), ctypes.c_uint32, P(abi.GsSpectrum),
+                                           P(P(abi.GsExitRecord)), P(ctypes.c_uint64), P(ctypes.c_uint64)]
+        L.gs_run_batch_records.restype = ctypes.c_int
         L.gs_set_batch_budget.argtypes = [vp, ctypes.c_uint64]
         L.gs_set_batch_budget.restype = ctypes.c_int
         L.gs_set_plain_limits.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64]
@@ -175,6 +178,22 @@ class Engine:
         rc = lib().gs_run_batch(self._ctx, arr, n, outs)
         self._check(rc, "gs_run_batch")
         return list(outs)
+
+    def run_batch_records(self, sims, rec_caps):
+        """(spectra, records per simulation, event counts); rec_caps[i] = 0
+        asks no records of simulation i."""
+        n = len(sims)
+        arr = (abi.GsParams * n)(*sims)
+        outs = (abi.GsSpectrum * n)()
+        bufs = [(abi.GsExitRecord * c)() if c else None for c in rec_caps]
+        ptrs = (ctypes.POINTER(abi.GsExitRecord) * n)(
+            *[ctypes.cast(b, ctypes.POINTER(abi.GsExitRecord)) if b is not None else None for b in bufs])
+        caps = (ctypes.c_uint64 * n)(*rec_caps)
+        nrec = (ctypes.c_uint64 * n)()
+        rc = lib().gs_run_batch_records(self._ctx, arr, n, outs, ptrs, caps, nrec)
+        self._check(rc, "gs_run_batch_records")
+        recs = [list(b[:min(c, k)]) if b is not None else None for b, c, k in zip(bufs, rec_caps, nrec)]
+        return list(outs), recs, list(nrec)
 
     def run_batch_status(self, sims):
         """Status code of a batch (no raise), for error-path tests."""

@@ -51,6 +51,18 @@ void usage() {
                  "              [--print-params]\n");
 }
 
+// The fields of a parameter block as JSON members, closing the object.
+void print_params_json(const gs_params &p) {
+    std::printf("\"model\": %d, \"injection\": %d, \"flags\": %u, \"K0\": %.17g, \"V\": %.17g, "
+                "\"dt\": %.17g, \"r_inj\": %.17g, \"theta_inj\": %.17g, \"tilt\": %.17g, "
+                "\"polarity\": %.17g, \"ratio\": %.17g, \"delta0\": %.17g, \"T_min\": %.17g, "
+                "\"T_max\": %.17g, \"grid_n\": %u, \"n_particles\": %llu, \"seed\": %llu, "
+                "\"n_bins\": %u, \"n_weights\": %u, \"weights\": [%d, %d, %d, %d]}\n",
+                p.model, p.injection, p.flags, p.K0, p.V, p.dt, p.r_inj, p.theta_inj, p.tilt, p.polarity, p.ratio,
+                p.delta0, p.T_min, p.T_max, p.grid_n, (unsigned long long)p.n_particles, (unsigned long long)p.seed,
+                p.n_bins, p.n_weights, p.weights[0], p.weights[1], p.weights[2], p.weights[3]);
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -186,10 +198,12 @@ int main(int argc, char **argv) {
             return 1;
         }
         if (print_params) {
-            for (const gsdrop::BatchRow &r : rows)
-                std::printf("{\"model\": \"%s\", \"K0\": %.17g, \"V\": %.17g, \"dt\": %.17g, \"N\": %ld, "
-                            "\"r\": %.17g, \"theta\": %.17g, \"name\": \"%s\", \"year\": %d, \"month\": %d}\n",
-                            r.model.c_str(), r.K0, r.V, r.dt, r.N, r.r, r.theta, r.name.c_str(), r.year, r.month);
+            // the parameter block and directory of every row that would run
+            const gsdrop::PreparedBatch B = gsdrop::prepare_batch_rows(pc, rows);
+            for (size_t i = 0; i < B.cfg.size(); ++i) {
+                std::printf("{\"row\": %zu, \"destination\": \"%s\", ", B.row[i] + 1, B.dest[i].c_str());
+                print_params_json(B.cfg[i].params);
+            }
             return 0;
         }
         const int n = gsdrop::run_batch_rows(pc, rows);
@@ -214,16 +228,8 @@ int main(int argc, char **argv) {
             std::fprintf(stderr, "[error] %s\n", err.c_str());
             return 1;
         }
-        const gs_params &p = cfg.params;
-        std::printf("{\"model\": %d, \"injection\": %d, \"flags\": %u, \"K0\": %.17g, \"V\": %.17g, "
-                    "\"dt\": %.17g, \"r_inj\": %.17g, \"theta_inj\": %.17g, \"tilt\": %.17g, "
-                    "\"polarity\": %.17g, \"ratio\": %.17g, \"delta0\": %.17g, \"T_min\": %.17g, "
-                    "\"T_max\": %.17g, \"grid_n\": %u, \"n_particles\": %llu, \"seed\": %llu, "
-                    "\"n_bins\": %u, \"n_weights\": %u, \"weights\": [%d, %d, %d, %d]}\n",
-                    p.model, p.injection, p.flags, p.K0, p.V, p.dt, p.r_inj, p.theta_inj, p.tilt, p.polarity,
-                    p.ratio, p.delta0, p.T_min, p.T_max, p.grid_n, (unsigned long long)p.n_particles,
-                    (unsigned long long)p.seed, p.n_bins, p.n_weights, p.weights[0], p.weights[1],
-                    p.weights[2], p.weights[3]);
+        std::printf("{");
+        print_params_json(cfg.params);
         return 0;
     }
     gsdrop::run_algorithm(model, pc);

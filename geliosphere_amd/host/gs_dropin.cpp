@@ -219,31 +219,42 @@ std::string directory_name(ParamsCarrier *pc) {
     return name;
 }
 
+std::string simulation_report(ParamsCarrier *pc, const RunConfig &cfg, const char *engine) {
+    std::string out;
+    char line[256];
+    auto add = [&](const char *fmt, auto... v) {
+        std::snprintf(line, sizeof line, fmt, v...);
+        out += line;
+    };
+    const std::string model = pc->getString("model", "1D Fp");
+    add("Model: %s\n", model.c_str());
+    out += "---------------------\n";
+    if (pc->getInt("month_option", -1) != -1) {
+        add("Selected month: %d\n", pc->getInt("month_option", -1));
+        add("Selected year: %d\n", pc->getInt("year_option", -1));
+    }
+    if (model == "2D SolarProp-like") add("SolarProp-like model ratio: %g\n", pc->getFloat("solarPropRatio", 0.02f));
+    if (model == "2D Geliosphere") {
+        add("Geliosphere model ratio: %g\n", pc->getFloat("geliosphereRatio", 0.2f));
+        add("C Delta: %g\n", pc->getFloat("C_delta", 8.7e-5f));
+        const float tilt = pc->getFloat("tilt_angle", pc->getFloat("default_tilt_angle", 0.1f));
+        add("Loaded tilt angle from file: %g\n", tilt);
+        add("Calculated tilt angle:: %g\n", tilt * 3.1415926535f / 180.0f);
+    }
+    add("dt: %g\n", pc->getFloat("dt", pc->getFloat("dt_default", 50.0f)));
+    add("K0: %g\n", pc->getFloat("K0", pc->getFloat("K0_default", 0.000222f)));
+    add("V: %g\n", pc->getFloat("V", pc->getFloat("V_default", 2.66667e-6f)));
+    add("Engine: %s\n", engine);
+    add("Seed: %llu\n", (unsigned long long)cfg.seed);
+    add("Test particles: %llu\n", (unsigned long long)cfg.params.n_particles);
+    return out;
+}
+
 void write_simulation_report(ParamsCarrier *pc, const RunConfig &cfg) {
     FILE *f = std::fopen("simulation_log.txt", "w");
     if (!f) return;
-    const std::string model = pc->getString("model", "1D Fp");
-    std::fprintf(f, "Model: %s\n", model.c_str());
-    std::fprintf(f, "---------------------\n");
-    if (pc->getInt("month_option", -1) != -1) {
-        std::fprintf(f, "Selected month: %d\n", pc->getInt("month_option", -1));
-        std::fprintf(f, "Selected year: %d\n", pc->getInt("year_option", -1));
-    }
-    if (model == "2D SolarProp-like")
-        std::fprintf(f, "SolarProp-like model ratio: %g\n", pc->getFloat("solarPropRatio", 0.02f));
-    if (model == "2D Geliosphere") {
-        std::fprintf(f, "Geliosphere model ratio: %g\n", pc->getFloat("geliosphereRatio", 0.2f));
-        std::fprintf(f, "C Delta: %g\n", pc->getFloat("C_delta", 8.7e-5f));
-        const float tilt = pc->getFloat("tilt_angle", pc->getFloat("default_tilt_angle", 0.1f));
-        std::fprintf(f, "Loaded tilt angle from file: %g\n", tilt);
-        std::fprintf(f, "Calculated tilt angle:: %g\n", tilt * 3.1415926535f / 180.0f);
-    }
-    std::fprintf(f, "dt: %g\n", pc->getFloat("dt", pc->getFloat("dt_default", 50.0f)));
-    std::fprintf(f, "K0: %g\n", pc->getFloat("K0", pc->getFloat("K0_default", 0.000222f)));
-    std::fprintf(f, "V: %g\n", pc->getFloat("V", pc->getFloat("V_default", 2.66667e-6f)));
-    std::fprintf(f, "Engine: gfx950 SDE trajectory engine (FP64, Philox4x32-10)\n");
-    std::fprintf(f, "Seed: %llu\n", (unsigned long long)cfg.seed);
-    std::fprintf(f, "Test particles: %llu\n", (unsigned long long)cfg.params.n_particles);
+    const std::string text = simulation_report(pc, cfg, "gfx950 SDE trajectory engine (FP64, Philox4x32-10)");
+    std::fputs(text.c_str(), f);
     std::fclose(f);
 }
 

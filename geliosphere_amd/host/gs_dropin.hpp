@@ -60,7 +60,9 @@ bool config_from_carrier(ParamsCarrier *pc, int model, RunConfig *cfg, std::stri
 // Working-directory contract (reference Utils/src/FileUtils.cpp:14-97).
 bool create_directory(const std::string &model_dir, const std::string &destination);
 std::string directory_name(ParamsCarrier *pc);
-// simulation_log.txt (FileUtils.cpp:99-125) plus the engine and seed.
+// simulation_log.txt (FileUtils.cpp:99-125) plus the engine and seed: its
+// text, and the file in the working directory.
+std::string simulation_report(ParamsCarrier *pc, const RunConfig &cfg, const char *engine);
 void write_simulation_report(ParamsCarrier *pc, const RunConfig &cfg);
 
 // log.dat lines in the reference's per-model formats
@@ -102,9 +104,22 @@ struct BatchRow {
 // Rows of a batch CSV with the header year,month,K0,V,dt,N,r,theta,
 // pathToCustomSettingsFile,name,model (README.md:39); empty cells are -1.
 bool read_batch_csv(const std::string &path, std::vector<BatchRow> *rows, std::string *err);
-// All rows in one gs_run_batch call; each row's histogram and report go to
-// results/<MODEL>/<name>/ (the directory its own run would use).  Returns
-// the number of rows run.
+// Rows turned into runs as the reference's per-row parsing fills the carrier
+// (BatchRun.cpp:254-342); rows that fail a check are skipped with a message.
+struct PreparedBatch {
+    std::vector<size_t> row;           // index of the row in the file
+    std::vector<int> model;
+    std::vector<RunConfig> cfg;
+    std::vector<std::string> dest;     // results/<MODEL>/<dest>/
+    std::vector<std::string> report;   // simulation_log.txt
+    std::vector<bool> remove_log;      // remove_log_files_after_simulation
+};
+PreparedBatch prepare_batch_rows(ParamsCarrier *pc, const std::vector<BatchRow> &rows);
+// All rows in one gs_run_batch_records call; each row then leaves in
+// results/<MODEL>/<dest>/ what its own run would (BatchRun.cpp:50-78 runs
+// each row's Algorithm): log.dat (removed after the Results pass unless
+// remove_log_files_after_simulation = 0), the Results pass's spectrum files,
+// simulation_log.txt, and gs_histogram.dat.  Returns the number of rows run.
 int run_batch_rows(ParamsCarrier *pc, const std::vector<BatchRow> &rows);
 
 // The settings file's values (Settings.toml schema, Input/src/TomlSettings.cpp:

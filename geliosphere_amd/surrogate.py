@@ -241,16 +241,30 @@ def generate(engine, sims, weight_slot=0, chunk=64):
     return np.array(X, dtype=np.float32), np.array(Y, dtype=np.float32), np.array(W, dtype=np.float32)
 
 
+# Models the ensemble serves.  1D F-p is Monte Carlo only: a 1 AU crossing
+# is rare (tens per 1e6 particles), so a training spectrum carries
+# percent-level noise in every bin and the members never agree to 1 %.
+SURROGATE_MODELS = (abi.MODEL_1D_BP, abi.MODEL_2D_SOLARPROP, abi.MODEL_2D_GELIOSPHERE)
+
+
 def query(surrogate, engine, sims):
     """Spectra targets for parameter blocks: surrogate mean where it is
-    trusted, Monte Carlo (batched engine) where the ensemble disagrees or the
-    query is outside the training box.  Returns (y [n][30], from_mc [n])."""
+    trusted, Monte Carlo (batched engine) where the ensemble disagrees, the
+    query is outside the training box or its model is not served by the
+    ensemble (1D F-p).  Returns (y [n][30], from_mc [n]); a bin that a Monte
+    Carlo run leaves with too few events to estimate is NaN."""
     x = np.array([features(p) for p in sims], dtype=np.float32)
     mean, _, fb = surrogate.predict(x, want_sd=False)
+    fb = fb.copy()
+    for i, p in enumerate(sims):
+        if p.model not in SURROGATE_MODELS:
+            fb[i] |= FALLBACK_DOMAIN
     y = mean.astype(np.float64)
     redo = [i for i in range(len(sims)) if fb[i]]
     if redo:
         outs = engine.run_batch([sims[i] for i in redo])
         for i, s in zip(redo, outs):
-            y[i], _ = targets(sims[i], s)
+            yi, wi = targets(sims[i], s)
+            yi[wi == 0] = np.nan
+            y[i] = yi
     return y, fb != 0

@@ -18,7 +18,9 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CLI = os.path.join(HERE, "gs_cli")
-FIXTURE = os.path.join(os.path.dirname(HERE), "tests", "golden", "ulysses_1994_1998.csv")
+# Per rotation: Ulysses' position and the KET P190 flux, made from the
+# reference's data files by data/make_ulysses_table.py.
+ROTATIONS = os.path.join(HERE, "data", "ulysses_1994_1998.csv")
 
 SETTINGS = """[default_values]
 K0 = 5e22
@@ -45,7 +47,7 @@ remove_log_files_after_simulation = true
 """
 
 
-def load_rotations(path=FIXTURE):
+def load_rotations(path=ROTATIONS):
     """(table header, rows) with row = dict(table_row, year, month, day, r, theta, ket_p116, ket_p190)."""
     with open(path) as f:
         header = f.readline()[2:].strip()

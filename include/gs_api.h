@@ -221,6 +221,15 @@ gs_status gs_run_plain_noalloc(gs_ctx *ctx, const gs_params *p, gs_spectrum *out
  * run alone.  gs_last_run_stats covers the whole batch. */
 gs_status gs_run_batch(gs_ctx *ctx, const gs_params *sims, uint32_t n_sims,
                        gs_spectrum *outs);
+/* gs_run_batch that also returns exit records: for simulation i with
+ * recs[i] != NULL and rec_caps[i] > 0, recs[i] receives up to rec_caps[i] of
+ * its records in gs_run_plain's order (by particle id, then event order);
+ * n_recs (may be NULL) receives every simulation's event count.  recs may be
+ * NULL (no records).  Replaces what each row of the reference's batch leaves
+ * for its Results pass: a full Algorithm run, log.dat included
+ * (Algorithm/src/BatchRun.cpp:50-78, OneDimensionBpModel.cu:186-199). */
+gs_status gs_run_batch_records(gs_ctx *ctx, const gs_params *sims, uint32_t n_sims, gs_spectrum *outs,
+                               gs_exit_record *const *recs, const uint64_t *rec_caps, uint64_t *n_recs);
 /* Event records (64 B each) a batch may hold on the device at once
  * (default 2^29); larger batches run as consecutive sub-batches. */
 gs_status gs_set_batch_budget(gs_ctx *ctx, uint64_t max_events);

@@ -170,9 +170,17 @@ def test_batch_csv_rows(tmp_path):
     f.write_text("year,month,K0,V,dt,N,r,theta,pathToCustomSettingsFile,name,model\n"
                  "-1,-1,5e22,400,500,1,1.0,90,,runA,1D Bp\n"
                  "2010,5,,,,2,30,45,,\"run B\",2D Geliosphere\n")
-    rows = [json.loads(l) for l in cli("-b", str(f), "--print-params").stdout.splitlines()]
-    assert rows[0]["model"] == "1D Bp" and rows[0]["K0"] == 5e22 and rows[0]["dt"] == 500 and rows[0]["name"] == "runA"
-    assert rows[1]["K0"] == -1 and rows[1]["N"] == 2 and rows[1]["name"] == "run B" and rows[1]["year"] == 2010
+    rows = [json.loads(l) for l in cli("-b", str(f), "--seed", "7", "--print-params").stdout.splitlines()]
+    assert [r["row"] for r in rows] == [1, 2] and [r["destination"] for r in rows] == ["runA", "run B"]
+    assert rows[0]["model"] == 1 and rows[0]["dt"] == 500 and rows[0]["K0"] == pytest.approx(conv_k0(5e22))
+    assert rows[0]["n_particles"] == 1 << 20 and rows[1]["n_particles"] == 2 << 20
+    assert rows[1]["model"] == 3 and rows[1]["r_inj"] == 30 and rows[1]["theta_inj"] == pytest.approx(deg_f32(45))
+    assert [r["seed"] for r in rows] == [7, 8]          # one particle stream per row
+    assert all(r["injection"] == 1 for r in rows)       # the batch defaults: uniform injection
+    f.write_text("year,month,K0,V,dt,N,r,theta,pathToCustomSettingsFile,name,model\n"
+                 "-1,-1,5e22,400,500,0,1.0,90,,bad,1D Bp\n-1,-1,5e22,400,500,1,1.0,90,,ok,1D Fp\n")
+    r = cli("-b", str(f), "--seed", "7", "--print-params")
+    assert [json.loads(l)["destination"] for l in r.stdout.splitlines()] == ["ok"] and "skipping" in r.stderr
 
 
 def test_model_flags_exclude_each_other():
@@ -410,7 +418,7 @@ def test_dropin_builds_against_the_reference_params_carrier(tmp_path):
     for args in (["-B", "--seed", "5"], ["-F", "-K", "3e22", "-V", "600", "-d", "100", "--seed", "1"],
                  ["-T", "-K", "1e22", "-V", "500", "--tilt", "40", "--polarity", "-1", "--seed", "2"],
                  ["-E", "-s", str(st), "--seed", "3"], ["--custom-model", "2D SolarProp-like", "--seed", "4"],
-                 ["-b", str(b)]):
+                 ["-b", str(b), "--seed", "9"]):
         ref = subprocess.run([str(exe), *args, "--print-params"], capture_output=True, text=True, timeout=60)
         own = cli(*args, "--print-params")
         assert ref.returncode == 0 and ref.stdout == own.stdout, args

@@ -183,3 +183,29 @@ def test_config4_sampled_batch_matches_oracle(engine, oracle):
             for k in range(4):
                 zs, chi2, dof, pval = compare_spectra(g, o, k=k, min_count=5)
                 assert all(abs(z) < 3.0 for z in zs), (i, k, zs)
+
+
+def test_batch_records_equal_plain_records(engine):
+    """gs_run_batch_records returns each simulation's exit records as
+    gs_run_plain does (sorted by particle id), including a simulation that
+    runs alone because the event budget is small, a simulation asking no
+    records, and a record capacity below the event count."""
+    sims = sample_sims(6, seed=8)
+    sims.append(abi.make_params(abi.MODEL_1D_BP, N=70_000, seed=81))   # alone under a 2^16 budget
+    caps = [2 * p.n_particles + 16 for p in sims]
+    caps[1] = 0                       # no records for this one
+    caps[2] = 7                       # fewer than its events
+    engine.set_batch_budget(1 << 16)
+    try:
+        outs, recs, nrec = engine.run_batch_records(sims, caps)
+    finally:
+        engine.set_batch_budget(1 << 29)
+    for i, p in enumerate(sims):
+        s, r, n = engine.run_plain(p, records=True, rec_cap=max(2 * p.n_particles, 16))
+        assert bytes(s) == bytes(outs[i]) and nrec[i] == n, i
+        if caps[i] == 0:
+            assert recs[i] is None
+            continue
+        want = r[:caps[i]]
+        assert len(recs[i]) == min(caps[i], n), i
+        assert [bytes(a) for a in recs[i]] == [bytes(b) for b in want], i

@@ -109,18 +109,78 @@ def test_cli_log_removed_by_default(engine, tmp_path):
     assert not (d / "log.dat").exists() and (d / "output_1e2bin.dat").exists()
 
 
+KEEP_LOG_SETTINGS = """[default_values]
+K0 = 5e22
+V = 400.0
+dt = 50.0
+r_injection = 1.0
+uniform_energy_injection_maximum = 101.0
+
+[2d_models_common_settings]
+theta_injection = 90.0
+use_uniform_injection = true
+
+[SolarProp_like_model_settings]
+SolarProp_ratio = 0.02
+
+[Geliosphere_model_settings]
+Geliosphere_ratio = 0.2
+K0_ratio = 1.0
+C_delta = 8.7e-5
+default_tilt_angle = 0.1
+
+[advanced_settings]
+remove_log_files_after_simulation = false
+"""
+
+REFERENCE_FILES = {0: ["output_1e2bin.dat", "output_1e3bin.dat", "output_4e2bin.dat", "output_logbin.dat"],
+                   1: ["output_1e2bin.dat", "output_1e3bin.dat", "output_4e2bin.dat", "output_logbin.dat"],
+                   2: ["Usoskin_2005.dat", "Weber.dat", "Burger.dat", "Ulysses.dat", "Ams.dat"],
+                   3: ["Usoskin_2005.dat", "Weber.dat", "Burger.dat", "Ulysses.dat", "Ams.dat"]}
+
+
 def test_cli_batch_rows(engine, tmp_path):
+    """gs_cli -b runs every row in one gs_run_batch_records call and leaves in
+    each row's directory what the row's own run leaves (the reference's
+    BatchRun runs each row's Algorithm, Algorithm/src/BatchRun.cpp:50-78):
+    log.dat (kept here by the row's settings file), the Results pass's
+    spectrum files, simulation_log.txt and gs_histogram.dat.  log.dat holds
+    exactly the records gs_run_plain returns for the row's parameter block,
+    and the histogram its spectrum."""
+    keep = tmp_path / "keep.toml"
+    keep.write_text(KEEP_LOG_SETTINGS)
     f = tmp_path / "batch.csv"
     f.write_text("year,month,K0,V,dt,N,r,theta,pathToCustomSettingsFile,name,model\n"
-                 "-1,-1,5e22,400,500,1,1.0,90,,rowA,1D Bp\n"
-                 "-1,-1,1e22,600,100,1,5.0,90,,rowB,1D Fp\n"
-                 "-1,-1,5e22,,50,1,1.0,60,,rowC,2D Geliosphere\n"
-                 "-1,-1,5e22,,50,1,2.0,120,,rowD,2D SolarProp-like\n")
-    cli(["-b", str(f), "--particles", "3000", "--seed", "40"], tmp_path)
-    for name, mdir in (("rowA", "1DBP"), ("rowB", "1DFP"), ("rowC", "Geliosphere2D"), ("rowD", "SOLARPROPLike")):
-        d = tmp_path / "results" / mdir / name
-        assert (d / "gs_histogram.dat").exists() and (d / "simulation_log.txt").exists(), name
-        rows = [l.split() for l in open(d / "gs_histogram.dat") if not l.startswith("#")]
-        assert len(rows) == 30
-        if mdir != "1DFP":   # every backward particle ends in one exit (or outside the bins)
-            assert 0 < sum(int(r[2]) for r in rows) <= 3000
+                 "-1,-1,5e22,400,500,1,1.0,90,%s,rowA,1D Bp\n"
+                 "-1,-1,1e22,600,100,1,5.0,90,%s,rowB,1D Fp\n"
+                 "-1,-1,5e22,,50,1,1.0,60,%s,rowC,2D Geliosphere\n"
+                 "-1,-1,5e22,,50,1,2.0,120,,rowD,2D SolarProp-like\n" % (keep, keep, keep))
+    args = ["-b", str(f), "--particles", "3000", "--seed", "40"]
+    prepared = [json.loads(l) for l in subprocess.run([CLI, *args, "--print-params"], capture_output=True,
+                                                      text=True, check=True).stdout.splitlines()]
+    assert [r["destination"] for r in prepared] == ["rowA", "rowB", "rowC", "rowD"]
+    cli(args, tmp_path)
+    for r in prepared:
+        model = r["model"]
+        d = tmp_path / "results" / {0: "1DFP", 1: "1DBP", 2: "SOLARPROPLike", 3: "Geliosphere2D"}[model] / \
+            r["destination"]
+        for name in REFERENCE_FILES[model] + ["gs_histogram.dat", "simulation_log.txt"]:
+            assert (d / name).exists(), (r["destination"], name)
+        assert (d / "log.dat").exists() == (r["destination"] != "rowD")   # rowD: batch defaults remove it
+        assert "batched" in (d / "simulation_log.txt").read_text()
+        p = default_params(model)
+        for k in ("injection", "flags", "K0", "V", "dt", "r_inj", "theta_inj", "tilt", "polarity", "ratio",
+                  "delta0", "T_min", "T_max", "grid_n", "n_particles", "seed", "n_bins", "n_weights"):
+            setattr(p, k, r[k])
+        for i, w in enumerate(r["weights"]):
+            p.weights[i] = w
+        spec, recs, n_ev = engine.run_plain(p, records=True, rec_cap=max(16, 2 * p.n_particles))
+        check_histogram(d / "gs_histogram.dat", spec)
+        if r["destination"] != "rowD":
+            lines = open(d / "log.dat").read().splitlines()
+            assert len(lines) == n_ev
+            for line, e in zip(lines, recs):
+                v = [float(x) for x in line.split()]
+                t_col = {1: 1, 0: None, 2: 0, 3: 0}[model]
+                if t_col is not None:
+                    assert close(v[t_col], e.t_inj), (line, e.t_inj)

@@ -4,8 +4,8 @@ drop-in CLI.  For each of the 67 Carrington rotations of 1994-1998 a
 Geliosphere run at Ulysses' position takes the rotation's K0, tilt and
 polarity from the measured-value table (-m/-y); its Ulysses-spectrum flux
 above 0.25 GeV is set beside the Ulysses KET P190 proton flux averaged over
-the rotation (tests/golden/ulysses_1994_1998.csv, made from the reference's
-data files by make_ulysses_fixture.py).
+the rotation (geliosphere_amd/data/ulysses_1994_1998.csv, made from the reference's
+data files by make_ulysses_table.py).
 
 The model is not fitted to these data, so the bar is physical, not
 numerical: the modelled flux within a factor 2 of the measured one (median
