@@ -104,6 +104,21 @@ class GsExitRecord(ctypes.Structure):
     ]
 
 
+KERNEL_PLAIN = 0
+KERNEL_BATCH = 1
+
+
+class GsLaunchShape(ctypes.Structure):
+    _fields_ = [
+        ("block_threads", ctypes.c_uint32),
+        ("blocks_per_cu", ctypes.c_uint32),
+        ("chunk", ctypes.c_uint32),
+    ]
+
+    def as_tuple(self):
+        return (self.block_threads, self.blocks_per_cu, self.chunk)
+
+
 class GsRunStats(ctypes.Structure):
     _fields_ = [
         ("kernel_ms", ctypes.c_double),

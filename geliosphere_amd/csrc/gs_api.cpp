@@ -69,8 +69,11 @@ struct gs_ctx {
     uint64_t cap_bcounts = 0;
     gs::SpillRec *d_spill = nullptr;     // plain runs: two halves of cap_spill records
     uint64_t cap_spill = 0;
-    uint32_t block_override[4] = {0, 0, 0, 0};
-    uint32_t bpcu_override[4] = {0, 0, 0, 0};
+    // launch shape per [plain, batch][model]; zero fields take the defaults
+    // (256 threads, the occupancy's blocks per CU, kChunk ids per grab)
+    gs_launch_shape shape[2][4] = {};
+    double tuned_rate[2][4] = {};   // particle-steps/s of the tuned shapes (0: not tuned)
+    std::string arch;               // gcnArchName of the device
     // the device's fixed scratch array (gs_run_plain_noalloc), looked up once
     // per device when the context is created
     void *scratch = nullptr;
@@ -336,13 +339,24 @@ void fill_spectrum(const gs_params &p, const uint64_t *count, const uint64_t *di
     out->n_acc_overflow = overflow;
 }
 
-unsigned grid_for(gs_ctx *ctx, int model, bool batch) {
-    int bpcu = (int)ctx->bpcu_override[model];
-    if (bpcu <= 0) {
-        if (gs_traj_occupancy(model, batch, &bpcu) != hipSuccess || bpcu <= 0) bpcu = 1;
-    }
-    return (unsigned)(bpcu * ctx->n_cu);
+// The launch shape a trajectory kernel runs with: the context's shape for
+// (kernel, model) with its zero fields filled in (256 threads; blocks per CU
+// from the kernel's occupancy, and never above it; kChunk ids per grab).
+struct Resolved {
+    unsigned block, bpcu, chunk, grid;
+};
+Resolved resolve_shape(const gs_ctx *ctx, int model, bool batch) {
+    const gs_launch_shape &sh = ctx->shape[batch ? 1 : 0][model];
+    Resolved R;
+    R.block = sh.block_threads ? sh.block_threads : (unsigned)gs::kBlockThreads;
+    int occ = 0;
+    if (gs_traj_occupancy(model, batch, R.block, &occ) != hipSuccess || occ <= 0) occ = 1;
+    R.bpcu = sh.blocks_per_cu && (int)sh.blocks_per_cu <= occ ? sh.blocks_per_cu : (unsigned)occ;
+    R.chunk = sh.chunk ? sh.chunk : (unsigned)gs::kChunk;
+    R.grid = R.bpcu * (unsigned)ctx->n_cu;
+    return R;
 }
+
 
 size_t fold_lds_bytes(uint32_t nb, uint32_t nw) {
     return (size_t)gs::lds_words((int)nb, (int)nw) * sizeof(uint64_t);
@@ -438,7 +452,7 @@ gs_status run_sub_batch(gs_ctx *ctx, const gs_params *sims, const std::vector<ui
         return est_cost(sims[a]) > est_cost(sims[b]);
     });
     // ---- launches: (model, first sim, count)
-    struct Launch { int model; uint32_t base, count; unsigned grid; size_t off_pos; };
+    struct Launch { int model; uint32_t base, count; unsigned grid, block, chunk; size_t off_pos; };
     std::vector<Launch> launches;
     for (size_t j = 0; j < m;) {
         size_t k = j;
@@ -447,7 +461,10 @@ gs_status run_sub_batch(gs_ctx *ctx, const gs_params *sims, const std::vector<ui
         L.model = sims[order[j]].model;
         L.base = (uint32_t)j;
         L.count = (uint32_t)(k - j);
-        L.grid = grid_for(ctx, L.model, true);
+        const Resolved R = resolve_shape(ctx, L.model, true);
+        L.grid = R.grid;
+        L.block = R.block;
+        L.chunk = R.chunk;
         L.off_pos = 0;
         launches.push_back(L);
         j = k;
@@ -515,6 +532,8 @@ gs_status run_sub_batch(gs_ctx *ctx, const gs_params *sims, const std::vector<ui
         key.push_back((uint64_t)L.model);
         key.push_back(L.count);
         key.push_back(L.grid);
+        key.push_back(L.block);
+        key.push_back(L.chunk);
     }
     HIP_TRY(ctx, hipMemcpyAsync(ctx->d_params, ctx->h_params.data(), m * sizeof(DevParams),
                                 hipMemcpyHostToDevice, ctx->stream));
@@ -552,6 +571,7 @@ gs_status run_sub_batch(gs_ctx *ctx, const gs_params *sims, const std::vector<ui
             O.order = nullptr;
             O.spill = nullptr; O.spill_count = nullptr; O.spill_cap = 0;
             O.resume = nullptr; O.resume_n = 0;
+            O.chunk = L.chunk;
             if (ordered) {
                 e = gs_launch_batch_order(ctx->d_params + L.base, L.count, ctx->d_meta + L.off_pos, nv[l],
                                           ctx->d_bcounts + 2 * (size_t)gs::kOrderSimBuckets * L.base,
@@ -567,7 +587,7 @@ gs_status run_sub_batch(gs_ctx *ctx, const gs_params *sims, const std::vector<ui
             O.ev_cap = ctx->d_meta + m + L.base;
             O.n_events = ctx->d_meta + 2 * m + L.base;
             e = gs_launch_traj(L.model, true, ctx->d_params + L.base, L.count, ctx->d_meta + L.off_pos, O,
-                               L.grid, sg);
+                               L.grid, L.block, sg);
         }
         for (int g = 0; g < 3 && e == hipSuccess; ++g) {
             if (!used[g]) continue;
@@ -579,6 +599,7 @@ gs_status run_sub_batch(gs_ctx *ctx, const gs_params *sims, const std::vector<ui
             O.queue = nullptr; O.cycles = nullptr; O.order = nullptr;
             O.spill = nullptr; O.spill_count = nullptr; O.spill_cap = 0;
             O.resume = nullptr; O.resume_n = 0;
+            O.chunk = 0;
             O.g_count = W.count; O.g_digits = W.digits; O.g_stats = W.stats;
             O.events = ctx->d_events;
             O.ev_off = ctx->d_meta; O.ev_cap = ctx->d_meta + m; O.n_events = ctx->d_meta + 2 * m;
@@ -633,6 +654,7 @@ gs_status run_sub_batch(gs_ctx *ctx, const gs_params *sims, const std::vector<ui
     for (size_t l = 0; l < nl; ++l) {
         acc.lane_cycles += hmeta[c_base + l] * (uint64_t)gs::kWave * gs::kCycleSteps;
         acc.grid_blocks += launches[l].grid;
+        acc.block_threads = launches[l].block;
     }
     return GS_OK;
 }
@@ -667,7 +689,8 @@ gs_status run_plain_core(gs_ctx *ctx, const gs_params &pp, gs_spectrum *out, gs_
     if (want_recs && !B.own) return fail(ctx, GS_ERR_INVALID_ARG, "exit records need the context's buffers");
     Layout L = layout(B.words, 1);
     HIP_TRY(ctx, gs_launch_zero(B.words, kWordsPerSim, ctx->stream));
-    const unsigned grid = grid_for(ctx, p->model, false);
+    const Resolved RS = resolve_shape(ctx, p->model, false);
+    const unsigned grid = RS.grid;
     std::vector<gs_exit_record> kept;   // records only (never with fixed buffers)
     uint64_t total_events = 0;
     double kernel_ms = 0.0;
@@ -680,7 +703,8 @@ gs_status run_plain_core(gs_ctx *ctx, const gs_params &pp, gs_spectrum *out, gs_
     O.order = nullptr;
     O.spill = nullptr; O.spill_count = nullptr; O.spill_cap = 0;
     O.resume = nullptr; O.resume_n = 0;
-    const uint64_t lanes = (uint64_t)grid * gs::kBlockThreads;
+    O.chunk = RS.chunk;
+    const uint64_t lanes = (uint64_t)grid * RS.block;
     // largest chunk: 2^27 particles, or what the fixed event region holds
     // (B-p and 2D models emit at most one event per particle)
     const uint64_t chunk_max = B.own ? ctx->plain_chunk
@@ -738,7 +762,7 @@ gs_status run_plain_core(gs_ctx *ctx, const gs_params &pp, gs_spectrum *out, gs_
         O.spill_count = B.meta + 4;
         O.spill_cap = compact ? ctx->cap_spill : 0;
         HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-        HIP_TRY(ctx, gs_launch_traj(q.model, false, B.params, 1, nullptr, O, grid, ctx->stream));
+        HIP_TRY(ctx, gs_launch_traj(q.model, false, B.params, 1, nullptr, O, grid, RS.block, ctx->stream));
         // continuations: the handed-on particles, packed into full waves
         for (int round = 0, cur = 0; compact; ++round, cur ^= 1) {
             uint64_t pending = 0;
@@ -754,9 +778,8 @@ gs_status run_plain_core(gs_ctx *ctx, const gs_params &pp, gs_spectrum *out, gs_
             O.spill = pending > kCompactMin && round + 1 < kCompactRounds
                           ? ctx->d_spill + (size_t)(cur ^ 1) * ctx->cap_spill : nullptr;
             O.spill_count = B.meta + 4 + (cur ^ 1);
-            const unsigned g2 = (unsigned)std::min<uint64_t>(grid, (pending + gs::kBlockThreads - 1) /
-                                                                      gs::kBlockThreads);
-            HIP_TRY(ctx, gs_launch_traj(q.model, false, B.params, 1, nullptr, O, g2, ctx->stream));
+            const unsigned g2 = (unsigned)std::min<uint64_t>(grid, (pending + RS.block - 1) / RS.block);
+            HIP_TRY(ctx, gs_launch_traj(q.model, false, B.params, 1, nullptr, O, g2, RS.block, ctx->stream));
         }
         O.resume = nullptr;
         O.spill = nullptr;
@@ -818,7 +841,7 @@ gs_status run_plain_core(gs_ctx *ctx, const gs_params &pp, gs_spectrum *out, gs_
     ctx->stats.max_particle_steps = H.stats[gs::ST_MAX_STEPS];
     ctx->stats.active_cycles = H.stats[gs::ST_STEPS];
     ctx->stats.grid_blocks = grid;
-    ctx->stats.block_threads = gs::kBlockThreads;
+    ctx->stats.block_threads = RS.block;
     if (out->n_acc_overflow) return fail(ctx, GS_ERR_OVERFLOW, "weight accumulator overflow");
     return GS_OK;
 }
@@ -860,6 +883,9 @@ gs_status gs_create(gs_ctx **out, int device) {
         return GS_ERR_HIP;
     }
     ctx->n_cu = prop.multiProcessorCount;
+    ctx->arch = prop.gcnArchName;
+    for (char &ch : ctx->arch)
+        if (ch == ' ') ch = '_';
     *out = ctx;
     return GS_OK;
 }
@@ -927,11 +953,36 @@ gs_status gs_validate_params(const gs_params *p, char *msg, uint32_t msg_len) {
 
 gs_status gs_set_launch(gs_ctx *ctx, int32_t model, uint32_t block_threads, uint32_t blocks_per_cu) {
     if (!ctx || model < 0 || model > 3) return GS_ERR_INVALID_ARG;
-    if (block_threads != 0 && block_threads != (uint32_t)gs::kBlockThreads)
-        return fail(ctx, GS_ERR_UNSUPPORTED, "block_threads must be 0 or %d", gs::kBlockThreads);
-    if (blocks_per_cu > 64) return fail(ctx, GS_ERR_INVALID_ARG, "blocks_per_cu above 64");
-    ctx->block_override[model] = block_threads;
-    ctx->bpcu_override[model] = blocks_per_cu;
+    for (int k = 0; k < 2; ++k) {
+        gs_launch_shape sh = ctx->shape[k][model];
+        sh.block_threads = block_threads;
+        sh.blocks_per_cu = blocks_per_cu;
+        const gs_status st = gs_set_launch_shape(ctx, model, k, &sh);
+        if (st != GS_OK) return st;
+    }
+    return GS_OK;
+}
+
+gs_status gs_set_launch_shape(gs_ctx *ctx, int32_t model, int32_t kernel, const gs_launch_shape *shape) {
+    if (!ctx || model < 0 || model > 3 || (kernel != GS_KERNEL_PLAIN && kernel != GS_KERNEL_BATCH) || !shape)
+        return GS_ERR_INVALID_ARG;
+    if (shape->block_threads != 0 && !gs_traj_block_supported(shape->block_threads))
+        return fail(ctx, GS_ERR_UNSUPPORTED, "block_threads must be 0, 128, 256 or 512");
+    if (shape->blocks_per_cu > 64) return fail(ctx, GS_ERR_INVALID_ARG, "blocks_per_cu above 64");
+    if (shape->chunk > 4096) return fail(ctx, GS_ERR_INVALID_ARG, "chunk above 4096");
+    ctx->shape[kernel][model] = *shape;
+    ctx->tuned_rate[kernel][model] = 0.0;
+    return GS_OK;
+}
+
+gs_status gs_get_launch_shape(const gs_ctx *ctx, int32_t model, int32_t kernel, gs_launch_shape *shape) {
+    if (!ctx || model < 0 || model > 3 || (kernel != GS_KERNEL_PLAIN && kernel != GS_KERNEL_BATCH) || !shape)
+        return GS_ERR_INVALID_ARG;
+    HIP_TRY(const_cast<gs_ctx *>(ctx), hipSetDevice(ctx->device));
+    const Resolved R = resolve_shape(ctx, model, kernel == GS_KERNEL_BATCH);
+    shape->block_threads = R.block;
+    shape->blocks_per_cu = R.bpcu;
+    shape->chunk = R.chunk;
     return GS_OK;
 }
 
@@ -943,33 +994,143 @@ gs_status gs_set_order(gs_ctx *ctx, int32_t mode) {
     return GS_OK;
 }
 
+gs_status gs_autotune_shape(gs_ctx *ctx, const gs_params *pilots, uint32_t n_pilots, int32_t kernel,
+                            uint32_t repeats, gs_launch_shape *best_shape, double *best_steps_per_s) {
+    if (!ctx || !pilots || n_pilots == 0) return GS_ERR_INVALID_ARG;
+    if (kernel != GS_KERNEL_PLAIN && kernel != GS_KERNEL_BATCH) return fail(ctx, GS_ERR_INVALID_ARG, "unknown kernel");
+    if (kernel == GS_KERNEL_PLAIN && n_pilots != 1) return fail(ctx, GS_ERR_INVALID_ARG, "plain tuning takes one pilot");
+    const int model = pilots[0].model;
+    for (uint32_t i = 0; i < n_pilots; ++i) {
+        std::string msg;
+        if (!check_params(&pilots[i], msg)) return fail(ctx, GS_ERR_INVALID_ARG, "invalid pilot %u: %s", i, msg.c_str());
+        if (pilots[i].model != model) return fail(ctx, GS_ERR_INVALID_ARG, "pilots of more than one model");
+    }
+    if (repeats == 0) repeats = 1;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const bool batch = kernel == GS_KERNEL_BATCH;
+    const gs_launch_shape saved = ctx->shape[kernel][model];
+    std::vector<gs_spectrum> outs(n_pilots);
+    // best rate (particle-steps/s of trajectory kernel time) of one shape
+    auto measure = [&](const gs_launch_shape &sh, double *rate) -> gs_status {
+        ctx->shape[kernel][model] = sh;
+        double best = 0.0;
+        for (uint32_t r = 0; r < repeats; ++r) {
+            const gs_status st = batch ? gs_run_batch(ctx, pilots, n_pilots, outs.data())
+                                       : gs_run_plain(ctx, pilots, outs.data(), nullptr, 0, nullptr);
+            if (st != GS_OK) return st;
+            uint64_t steps = 0;
+            for (const gs_spectrum &o : outs) steps += o.total_steps;
+            const double v = ctx->stats.kernel_ms > 0 ? (double)steps / (ctx->stats.kernel_ms * 1e-3) : 0.0;
+            best = std::max(best, v);
+        }
+        *rate = best;
+        return GS_OK;
+    };
+    gs_launch_shape best{256, 0, (uint32_t)gs::kChunk};
+    double best_rate = -1.0;
+    auto consider = [&](const gs_launch_shape &sh) -> gs_status {
+        double v = 0.0;
+        const gs_status st = measure(sh, &v);
+        if (st == GS_OK && v > best_rate) { best_rate = v; best = sh; }
+        return st;
+    };
+    gs_status st = GS_OK;
+    // 1. blocks per CU at 256 threads and the default grab
+    int occ = 0;
+    HIP_TRY(ctx, gs_traj_occupancy(model, batch, 256, &occ));
+    for (int b = 1; b <= std::max(occ, 1) && st == GS_OK; ++b) st = consider({256, (uint32_t)b, (uint32_t)gs::kChunk});
+    // 2. the other workgroup sizes at about the same waves per CU, and at their occupancy
+    const unsigned waves = best.blocks_per_cu * 4;
+    for (unsigned blk : {128u, 512u}) {
+        int o = 0;
+        HIP_TRY(ctx, gs_traj_occupancy(model, batch, blk, &o));
+        const int eq = (int)((waves * 64 + blk / 2) / blk);
+        std::vector<int> cand = {eq - 1, eq, eq + 1, o};
+        std::sort(cand.begin(), cand.end());
+        cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+        for (int b : cand)
+            if (b >= 1 && b <= o && st == GS_OK) st = consider({blk, (uint32_t)b, (uint32_t)gs::kChunk});
+    }
+    // 3. ids per queue grab
+    const gs_launch_shape base = best;
+    for (uint32_t c : {16u, 32u, 128u, 256u})
+        if (st == GS_OK) st = consider({base.block_threads, base.blocks_per_cu, c});
+    if (st != GS_OK) {
+        ctx->shape[kernel][model] = saved;
+        return st;
+    }
+    ctx->shape[kernel][model] = best;
+    ctx->tuned_rate[kernel][model] = best_rate;
+    if (best_shape) *best_shape = best;
+    if (best_steps_per_s) *best_steps_per_s = best_rate;
+    return GS_OK;
+}
+
 gs_status gs_autotune(gs_ctx *ctx, const gs_params *pilot, uint32_t repeats, uint32_t *best_bpcu,
                       double *best_steps_per_s) {
-    if (!ctx) return GS_ERR_INVALID_ARG;
-    std::string msg;
-    if (!check_params(pilot, msg)) return fail(ctx, GS_ERR_INVALID_ARG, "invalid pilot: %s", msg.c_str());
-    if (repeats == 0) repeats = 1;
-    const int model = pilot->model;
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    int max_bpcu = 0;
-    HIP_TRY(ctx, gs_traj_occupancy(model, false, &max_bpcu));
-    if (max_bpcu <= 0) max_bpcu = 1;
-    const uint32_t saved = ctx->bpcu_override[model];
-    gs_spectrum out;
-    double best = -1.0;
-    uint32_t best_b = 0;
-    for (int b = 1; b <= max_bpcu; ++b) {
-        ctx->bpcu_override[model] = (uint32_t)b;
-        for (uint32_t r = 0; r < repeats; ++r) {
-            const gs_status st = gs_run_plain(ctx, pilot, &out, nullptr, 0, nullptr);
-            if (st != GS_OK) { ctx->bpcu_override[model] = saved; return st; }
-            const double rate = ctx->stats.kernel_ms > 0 ? (double)out.total_steps / (ctx->stats.kernel_ms * 1e-3) : 0.0;
-            if (rate > best) { best = rate; best_b = (uint32_t)b; }
+    gs_launch_shape sh;
+    const gs_status st = gs_autotune_shape(ctx, pilot, 1, GS_KERNEL_PLAIN, repeats, &sh, best_steps_per_s);
+    if (st == GS_OK && best_bpcu) *best_bpcu = sh.blocks_per_cu;
+    return st;
+}
+
+// Cache lines: "<arch> <CUs> <plain|batch> <model> <block_threads> <blocks_per_cu> <chunk> <steps/s>".
+gs_status gs_autotune_save(const gs_ctx *ctx, const char *path) {
+    if (!ctx || !path) return GS_ERR_INVALID_ARG;
+    std::vector<std::string> lines;
+    if (FILE *in = std::fopen(path, "r")) {   // keep other devices' and untuned entries
+        char buf[512];
+        while (std::fgets(buf, sizeof buf, in)) {
+            char arch[256], kind[16];
+            int cus, model;
+            if (std::sscanf(buf, "%255s %d %15s %d", arch, &cus, kind, &model) == 4 && ctx->arch == arch &&
+                cus == ctx->n_cu && model >= 0 && model < 4) {
+                const int k = std::strcmp(kind, "batch") == 0 ? 1 : 0;
+                if (ctx->tuned_rate[k][model] > 0) continue;   // replaced below
+            }
+            lines.emplace_back(buf);
         }
+        std::fclose(in);
     }
-    ctx->bpcu_override[model] = best_b;
-    if (best_bpcu) *best_bpcu = best_b;
-    if (best_steps_per_s) *best_steps_per_s = best;
+    FILE *out = std::fopen(path, "w");
+    if (!out) return fail(const_cast<gs_ctx *>(ctx), GS_ERR_IO, "cannot write %s", path);
+    if (lines.empty()) std::fputs("# arch CUs kernel model block_threads blocks_per_cu chunk particle_steps_per_s\n", out);
+    for (const std::string &l : lines) std::fputs(l.c_str(), out);
+    for (int k = 0; k < 2; ++k)
+        for (int m = 0; m < 4; ++m)
+            if (ctx->tuned_rate[k][m] > 0) {
+                const gs_launch_shape &sh = ctx->shape[k][m];
+                std::fprintf(out, "%s %d %s %d %u %u %u %.6g\n", ctx->arch.c_str(), ctx->n_cu, k ? "batch" : "plain",
+                             m, sh.block_threads, sh.blocks_per_cu, sh.chunk, ctx->tuned_rate[k][m]);
+            }
+    std::fclose(out);
+    return GS_OK;
+}
+
+gs_status gs_autotune_load(gs_ctx *ctx, const char *path, uint32_t *n_applied) {
+    if (!ctx || !path) return GS_ERR_INVALID_ARG;
+    if (n_applied) *n_applied = 0;
+    FILE *in = std::fopen(path, "r");
+    if (!in) return fail(ctx, GS_ERR_IO, "cannot read %s", path);
+    char buf[512];
+    uint32_t n = 0;
+    while (std::fgets(buf, sizeof buf, in)) {
+        char arch[256], kind[16];
+        int cus, model;
+        unsigned blk, bpcu, chunk;
+        double rate;
+        if (std::sscanf(buf, "%255s %d %15s %d %u %u %u %lf", arch, &cus, kind, &model, &blk, &bpcu, &chunk,
+                        &rate) != 8)
+            continue;
+        if (ctx->arch != arch || cus != ctx->n_cu || model < 0 || model > 3) continue;
+        const int k = std::strcmp(kind, "batch") == 0 ? 1 : 0;
+        const gs_launch_shape sh{blk, bpcu, chunk};
+        if (gs_set_launch_shape(ctx, model, k, &sh) != GS_OK) continue;
+        ctx->tuned_rate[k][model] = rate;
+        ++n;
+    }
+    std::fclose(in);
+    if (n_applied) *n_applied = n;
     return GS_OK;
 }
 
@@ -1117,7 +1278,7 @@ gs_status gs_run_batch_records(gs_ctx *ctx, const gs_params *sims, uint32_t n_si
     const auto t_start = std::chrono::steady_clock::now();
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     gs_run_stats acc{};
-    acc.block_threads = gs::kBlockThreads;
+    acc.block_threads = 0;
     // Simulations too large for one event region run on their own.
     std::vector<uint32_t> small;
     for (uint32_t i = 0; i < n_sims; ++i) {

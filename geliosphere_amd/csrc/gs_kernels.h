@@ -19,9 +19,9 @@ constexpr double kGeC = (5.0 / 3.4) / 3.0;           // K_par factor (Geliospher
 constexpr double kGeKf = 1.0 / (3.0 * (5.0 / 3.4));  // derivative factor (GeliosphereCpuModel.cpp:210-299)
 
 constexpr int kWave = 64;
-constexpr int kBlockThreads = 256;
+constexpr int kBlockThreads = 256;  // default workgroup size (trajectory kernels: 128, 256 or 512)
 constexpr int kCycleSteps = 4;     // steps between refills (whole Philox blocks for every model)
-constexpr uint64_t kChunk = 64;    // particle ids taken from the global queue per grab
+constexpr uint64_t kChunk = 64;    // default particle ids a wave takes from the global queue per grab
 constexpr int kBatchMaxSims = 1024; // simulations per batch trajectory launch (offsets in LDS)
 
 // Indices of the statistics words (LDS and global).
@@ -144,6 +144,10 @@ struct DevOut {
     uint64_t spill_cap;
     const SpillRec *resume;
     uint64_t resume_n;
+    // Particle ids a wave takes from the queue per grab (0: kChunk): larger
+    // grabs mean fewer queue atomics, smaller ones a finer share-out of the
+    // last particles among the waves.
+    uint32_t chunk;
 };
 
 // LDS layout per workgroup: count[n_bins] | digits[n_weights][2][n_bins][16] | stat[ST_COUNT_]
@@ -174,14 +178,16 @@ hipError_t gs_scratch_address(int device, void **ptr, size_t *bytes);
 hipError_t gs_launch_zero(uint64_t *p, size_t n, hipStream_t s);
 hipError_t gs_launch_debug(int fn, const double *x, double *y, uint64_t n, uint64_t seed,
                            uint64_t pid, hipStream_t s);
-// Blocks of `kBlockThreads` a CU holds for the trajectory kernel of `model`.
-hipError_t gs_traj_occupancy(int model, bool batch, int *blocks_per_cu);
+// Workgroup sizes the trajectory kernels are built for.
+bool gs_traj_block_supported(unsigned block_threads);
+// Blocks of `block_threads` a CU holds for the trajectory kernel of `model`.
+hipError_t gs_traj_occupancy(int model, bool batch, unsigned block_threads, int *blocks_per_cu);
 // Trajectories of n_sims simulations of one model.  sims: device array;
 // offsets: device array of n_sims + 1 prefix sums of n_particles (the
 // virtual id space).  batch = false requires n_sims == 1 (uniform parameters).
 hipError_t gs_launch_traj(int model, bool batch, const gs::DevParams *sims, uint32_t n_sims,
                           const uint64_t *offsets, const gs::DevOut &O, unsigned grid,
-                          hipStream_t s);
+                          unsigned block_threads, hipStream_t s);
 // Scheduling order of the n = sims->n_particles particles of one plain
 // simulation: perm[0 .. n) receives the offsets 0 .. n-1 in ascending bucket
 // order (within a bucket in no fixed order).  counts: 2 * key.nb words of

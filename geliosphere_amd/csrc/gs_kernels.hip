@@ -67,13 +67,15 @@ constexpr int kStage = 32;         // records per wave
 constexpr int kStageFlush = 16;
 constexpr int kRecWords = (int)(sizeof(DevEvent) / sizeof(uint64_t));
 
+template <int kWaves>
 struct Stage {
-    uint64_t words[kBlockThreads / kWave][kStage * kRecWords];
-    uint32_t count[kBlockThreads / kWave];
+    uint64_t words[kWaves][kStage * kRecWords];
+    uint32_t count[kWaves];
 };
 
 // `sim` is the record's simulation within the launch (0 for plain runs).
-__device__ __forceinline__ void stage_event(Stage *st, const DevOut &O, uint64_t pid, uint32_t sim, double t_inj,
+template <class St>
+__device__ __forceinline__ void stage_event(St *st, const DevOut &O, uint64_t pid, uint32_t sim, double t_inj,
                                             double t_exit, double r, double theta, double sumac) {
     const int w = (int)(threadIdx.x / kWave);
     const uint32_t slot = atomicAdd(&st->count[w], 1u);
@@ -95,8 +97,8 @@ __device__ __forceinline__ void stage_event(Stage *st, const DevOut &O, uint64_t
 // Called by the whole wave (uniform control flow).  In a batch the staged
 // records normally share one simulation: they then go out as one range of
 // its region; otherwise each goes out on its own.
-template <bool kBatch>
-__device__ __forceinline__ void stage_flush(Stage *st, const DevOut &O, uint32_t min_count) {
+template <bool kBatch, class St>
+__device__ __forceinline__ void stage_flush(St *st, const DevOut &O, uint32_t min_count) {
     const int w = (int)(threadIdx.x / kWave), lane = (int)(threadIdx.x & (kWave - 1));
     __builtin_amdgcn_wave_barrier();
     uint32_t c = st->count[w];
@@ -154,9 +156,9 @@ __device__ __forceinline__ void flush_sim_stats(const DevOut &O, uint32_t sim, u
 //   the old one and loads the new one's step parameters, Philox key and step
 //   cap into registers.  A wave only ever moves forward in the id space, so a
 //   lane's simulation index never decreases.
-template <class Model, bool kBatch>
+template <class Model, bool kBatch, class St>
 __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, const uint64_t *off,
-                          uint64_t *stat, Stage *stage, const BmTables &bm, const DevOut &O) {
+                          uint64_t *stat, St *stage, const BmTables &bm, const DevOut &O) {
     constexpr int NPS = Model::kNormals;   // normals per step; a cycle uses NPS Philox blocks
     unsigned long long *queue = (unsigned long long *)O.queue;
     const int lane = (int)(threadIdx.x & (kWave - 1));
@@ -164,6 +166,7 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
     const DevParams &P0 = sims[0];
     const bool resume = !kBatch && O.resume != nullptr;
     const uint64_t N = kBatch ? off[n_sims] : (resume ? O.resume_n : P0.n_particles);
+    const uint64_t grab = O.chunk ? (uint64_t)O.chunk : kChunk;   // ids per queue grab (wave-uniform)
 
     // Per-simulation values (wave-uniform for plain runs, per lane in batches).
     uint32_t k0 = (uint32_t)P0.seed, k1 = (uint32_t)(P0.seed >> 32);
@@ -209,13 +212,13 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
             if (!active && (uint64_t)rank < avail) take = q_next + rank;
             if (avail < (uint64_t)n_idle) {
                 unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(queue, (unsigned long long)kChunk);
+                if (lane == 0) base = atomicAdd(queue, (unsigned long long)grab);
                 base = __shfl(base, 0);
                 if (base >= N) {
                     exhausted = true;
                     q_next = q_end = N;
                 } else {
-                    const uint64_t end = (base + kChunk < N) ? base + kChunk : N;
+                    const uint64_t end = (base + grab < N) ? base + grab : N;
                     const uint64_t need = (uint64_t)n_idle - avail;
                     if (!active && (uint64_t)rank >= avail) {
                         const uint64_t cand = base + ((uint64_t)rank - avail);
@@ -457,15 +460,16 @@ __device__ __forceinline__ void zero_lds(uint64_t *lds, int words) {
     for (int i = threadIdx.x; i < words; i += blockDim.x) lds[i] = 0;
 }
 
-// kMinWaves: waves per SIMD asked of the register allocator (1: no request).
-template <class Model, int kMinWaves>
-__global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(kMinWaves)))
+// kMinWaves: waves per SIMD asked of the register allocator (1: no request);
+// kThreads: the workgroup size (128, 256 or 512).
+template <class Model, int kMinWaves, int kThreads>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWaves)))
 plain_kernel(const DevParams *__restrict__ Pp, const DevOut O) {
     __shared__ uint64_t stat[ST_COUNT_];
-    __shared__ Stage stage;
+    __shared__ Stage<kThreads / kWave> stage;
     __shared__ BmTables bm;
     if (threadIdx.x < ST_COUNT_) stat[threadIdx.x] = 0;
-    if (threadIdx.x < kBlockThreads / kWave) stage.count[threadIdx.x] = 0;
+    if (threadIdx.x < kThreads / kWave) stage.count[threadIdx.x] = 0;
     load_bm_tables(&bm);
     __syncthreads();
     run_lanes<Model, false>(Pp, 1, nullptr, stat, &stage, bm, O);
@@ -481,16 +485,15 @@ plain_kernel(const DevParams *__restrict__ Pp, const DevOut O) {
 // Lanes hold their simulation's step parameters in VGPRs; kMinWaves asks the
 // register allocator for that many waves per SIMD (2D models: 2, trading a
 // few spills for occupancy).
-template <class Model, int kMinWaves>
-__global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(kMinWaves))) batch_kernel(const DevParams *__restrict__ sims,
-                                                              uint32_t n_sims,
-                                                              const uint64_t *__restrict__ offsets,
-                                                              const DevOut O) {
+template <class Model, int kMinWaves, int kThreads>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWaves)))
+batch_kernel(const DevParams *__restrict__ sims, uint32_t n_sims, const uint64_t *__restrict__ offsets,
+             const DevOut O) {
     __shared__ uint64_t off[kBatchMaxSims + 1];
-    __shared__ Stage stage;
+    __shared__ Stage<kThreads / kWave> stage;
     __shared__ BmTables bm;
     for (uint32_t i = threadIdx.x; i <= n_sims; i += blockDim.x) off[i] = offsets[i];
-    if (threadIdx.x < kBlockThreads / kWave) stage.count[threadIdx.x] = 0;
+    if (threadIdx.x < kThreads / kWave) stage.count[threadIdx.x] = 0;
     load_bm_tables(&bm);
     __syncthreads();
     run_lanes<Model, true>(sims, n_sims, off, nullptr, &stage, bm, O);
@@ -767,17 +770,36 @@ constexpr int kBatchWaves1D = 1, kBatchWavesBp = GS_BATCH_WAVES_BP, kBatchWavesS
 constexpr int kPlainWavesBp = GS_PLAIN_WAVES_BP;
 constexpr int kPlainWaves1D = GS_PLAIN_WAVES_1D, kPlainWavesSp = GS_PLAIN_WAVES_SP, kPlainWavesGe = GS_PLAIN_WAVES_GE;
 
-static const void *traj_kernel_ptr(int model, bool batch) {
+// A batch workgroup holds 8 KB of simulation offsets in LDS: at 128 threads
+// the LDS allows 4 waves per SIMD at most, so no more are asked for.
+constexpr int batch_waves(int waves, int threads) { return threads == 128 && waves > 4 ? 4 : waves; }
+
+template <class Model, int kBatchWaves, int kPlainWaves, int kThreads>
+static const void *traj_kernel_ptr_t(bool batch) {
+    return batch ? (const void *)&batch_kernel<Model, batch_waves(kBatchWaves, kThreads), kThreads>
+                 : (const void *)&plain_kernel<Model, kPlainWaves, kThreads>;
+}
+
+template <int kThreads>
+static const void *traj_kernel_ptr_b(int model, bool batch) {
     switch (model) {
-    case GS_MODEL_1D_FP:
-        return batch ? (const void *)&batch_kernel<Fp, kBatchWaves1D> : (const void *)&plain_kernel<Fp, kPlainWaves1D>;
-    case GS_MODEL_1D_BP:
-        return batch ? (const void *)&batch_kernel<Bp, kBatchWavesBp> : (const void *)&plain_kernel<Bp, kPlainWavesBp>;
-    case GS_MODEL_2D_SOLARPROP:
-        return batch ? (const void *)&batch_kernel<Sp, kBatchWavesSp> : (const void *)&plain_kernel<Sp, kPlainWavesSp>;
-    default:
-        return batch ? (const void *)&batch_kernel<Ge, kBatchWavesGe> : (const void *)&plain_kernel<Ge, kPlainWavesGe>;
+    case GS_MODEL_1D_FP: return traj_kernel_ptr_t<Fp, kBatchWaves1D, kPlainWaves1D, kThreads>(batch);
+    case GS_MODEL_1D_BP: return traj_kernel_ptr_t<Bp, kBatchWavesBp, kPlainWavesBp, kThreads>(batch);
+    case GS_MODEL_2D_SOLARPROP: return traj_kernel_ptr_t<Sp, kBatchWavesSp, kPlainWavesSp, kThreads>(batch);
+    default: return traj_kernel_ptr_t<Ge, kBatchWavesGe, kPlainWavesGe, kThreads>(batch);
     }
+}
+
+static const void *traj_kernel_ptr(int model, bool batch, unsigned block_threads) {
+    switch (block_threads) {
+    case 128: return traj_kernel_ptr_b<128>(model, batch);
+    case 512: return traj_kernel_ptr_b<512>(model, batch);
+    default: return traj_kernel_ptr_b<256>(model, batch);
+    }
+}
+
+bool gs_traj_block_supported(unsigned block_threads) {
+    return block_threads == 128 || block_threads == 256 || block_threads == 512;
 }
 
 hipError_t gs_launch_debug(int fn, const double *x, double *y, uint64_t n, uint64_t seed,
@@ -795,33 +817,56 @@ hipError_t gs_launch_zero(uint64_t *p, size_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t gs_traj_occupancy(int model, bool batch, int *blocks_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, traj_kernel_ptr(model, batch),
-                                                        kBlockThreads, 0);
+hipError_t gs_traj_occupancy(int model, bool batch, unsigned block_threads, int *blocks_per_cu) {
+    if (!gs_traj_block_supported(block_threads)) return hipErrorInvalidValue;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, traj_kernel_ptr(model, batch, block_threads),
+                                                        (int)block_threads, 0);
 }
 
-template <class Model, int kBatchWaves, int kPlainWaves>
+template <class Model, int kBatchWaves, int kPlainWaves, int kThreads>
 static void launch_traj_t(bool batch, const DevParams *sims, uint32_t n_sims, const uint64_t *offsets,
                           const DevOut &O, unsigned grid, hipStream_t s) {
     if (batch)
-        hipLaunchKernelGGL((batch_kernel<Model, kBatchWaves>), dim3(grid), dim3(kBlockThreads), 0, s, sims, n_sims,
-                           offsets, O);
+        hipLaunchKernelGGL((batch_kernel<Model, batch_waves(kBatchWaves, kThreads), kThreads>), dim3(grid),
+                           dim3(kThreads), 0, s, sims, n_sims, offsets, O);
     else
-        hipLaunchKernelGGL((plain_kernel<Model, kPlainWaves>), dim3(grid), dim3(kBlockThreads), 0, s, sims, O);
+        hipLaunchKernelGGL((plain_kernel<Model, kPlainWaves, kThreads>), dim3(grid), dim3(kThreads), 0, s, sims, O);
+}
+
+template <int kThreads>
+static bool launch_traj_b(int model, bool batch, const DevParams *sims, uint32_t n_sims, const uint64_t *offsets,
+                          const DevOut &O, unsigned grid, hipStream_t s) {
+    switch (model) {
+    case GS_MODEL_1D_FP:
+        launch_traj_t<Fp, kBatchWaves1D, kPlainWaves1D, kThreads>(batch, sims, n_sims, offsets, O, grid, s);
+        return true;
+    case GS_MODEL_1D_BP:
+        launch_traj_t<Bp, kBatchWavesBp, kPlainWavesBp, kThreads>(batch, sims, n_sims, offsets, O, grid, s);
+        return true;
+    case GS_MODEL_2D_SOLARPROP:
+        launch_traj_t<Sp, kBatchWavesSp, kPlainWavesSp, kThreads>(batch, sims, n_sims, offsets, O, grid, s);
+        return true;
+    case GS_MODEL_2D_GELIOSPHERE:
+        launch_traj_t<Ge, kBatchWavesGe, kPlainWavesGe, kThreads>(batch, sims, n_sims, offsets, O, grid, s);
+        return true;
+    default: return false;
+    }
 }
 
 hipError_t gs_launch_traj(int model, bool batch, const DevParams *sims, uint32_t n_sims,
-                          const uint64_t *offsets, const DevOut &O, unsigned grid, hipStream_t s) {
+                          const uint64_t *offsets, const DevOut &O, unsigned grid, unsigned block_threads,
+                          hipStream_t s) {
     if (n_sims == 0 || grid == 0) return hipSuccess;
     if ((!batch && n_sims != 1) || (batch && (n_sims > (uint32_t)kBatchMaxSims || !offsets)))
         return hipErrorInvalidValue;
-    switch (model) {
-    case GS_MODEL_1D_FP: launch_traj_t<Fp, kBatchWaves1D, kPlainWaves1D>(batch, sims, n_sims, offsets, O, grid, s); break;
-    case GS_MODEL_1D_BP: launch_traj_t<Bp, kBatchWavesBp, kPlainWavesBp>(batch, sims, n_sims, offsets, O, grid, s); break;
-    case GS_MODEL_2D_SOLARPROP: launch_traj_t<Sp, kBatchWavesSp, kPlainWavesSp>(batch, sims, n_sims, offsets, O, grid, s); break;
-    case GS_MODEL_2D_GELIOSPHERE: launch_traj_t<Ge, kBatchWavesGe, kPlainWavesGe>(batch, sims, n_sims, offsets, O, grid, s); break;
-    default: return hipErrorInvalidValue;
+    bool ok;
+    switch (block_threads) {
+    case 128: ok = launch_traj_b<128>(model, batch, sims, n_sims, offsets, O, grid, s); break;
+    case 256: ok = launch_traj_b<256>(model, batch, sims, n_sims, offsets, O, grid, s); break;
+    case 512: ok = launch_traj_b<512>(model, batch, sims, n_sims, offsets, O, grid, s); break;
+    default: ok = false; break;
     }
+    if (!ok) return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -16,8 +16,9 @@ _lib = None
 EXPORTS = ("gs_abi_version", "gs_create", "gs_destroy", "gs_last_error", "gs_default_params",
            "gs_validate_params", "gs_run_plain", "gs_run_plain_noalloc", "gs_run_batch", "gs_run_batch_records", "gs_set_batch_budget",
            "gs_set_plain_limits",
-           "gs_set_launch", "gs_set_order", "gs_autotune", "gs_last_run_stats", "gs_debug_math", "gs_debug_normals",
-           "gs_host_math")
+           "gs_set_launch", "gs_set_launch_shape", "gs_get_launch_shape", "gs_set_order", "gs_autotune",
+           "gs_autotune_shape", "gs_autotune_save", "gs_autotune_load", "gs_last_run_stats", "gs_debug_math",
+           "gs_debug_normals", "gs_host_math")
 
 ORDER_ID, ORDER_ENERGY = 0, 1
 
@@ -74,6 +75,17 @@ def lib():
         L.gs_autotune.argtypes = [vp, P(abi.GsParams), ctypes.c_uint32, P(ctypes.c_uint32),
                                   P(ctypes.c_double)]
         L.gs_autotune.restype = ctypes.c_int
+        L.gs_set_launch_shape.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, P(abi.GsLaunchShape)]
+        L.gs_set_launch_shape.restype = ctypes.c_int
+        L.gs_get_launch_shape.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, P(abi.GsLaunchShape)]
+        L.gs_get_launch_shape.restype = ctypes.c_int
+        L.gs_autotune_shape.argtypes = [vp, P(abi.GsParams), ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint32,
+                                        P(abi.GsLaunchShape), P(ctypes.c_double)]
+        L.gs_autotune_shape.restype = ctypes.c_int
+        L.gs_autotune_save.argtypes = [vp, ctypes.c_char_p]
+        L.gs_autotune_save.restype = ctypes.c_int
+        L.gs_autotune_load.argtypes = [vp, ctypes.c_char_p, P(ctypes.c_uint32)]
+        L.gs_autotune_load.restype = ctypes.c_int
         L.gs_last_run_stats.argtypes = [vp, P(abi.GsRunStats)]
         L.gs_last_run_stats.restype = ctypes.c_int
         dp = P(ctypes.c_double)
@@ -211,6 +223,36 @@ class Engine:
 
     def set_launch(self, model, block_threads=0, blocks_per_cu=0):
         self._check(lib().gs_set_launch(self._ctx, model, block_threads, blocks_per_cu), "gs_set_launch")
+
+    def set_launch_shape(self, model, kernel, block_threads=0, blocks_per_cu=0, chunk=0):
+        sh = abi.GsLaunchShape(block_threads, blocks_per_cu, chunk)
+        self._check(lib().gs_set_launch_shape(self._ctx, model, kernel, ctypes.byref(sh)), "gs_set_launch_shape")
+
+    def launch_shape(self, model, kernel):
+        """(block_threads, blocks_per_cu, chunk) a launch of the kernel would use."""
+        sh = abi.GsLaunchShape()
+        self._check(lib().gs_get_launch_shape(self._ctx, model, kernel, ctypes.byref(sh)), "gs_get_launch_shape")
+        return sh.as_tuple()
+
+    def autotune_shape(self, pilots, kernel, repeats=1):
+        """Tune one kernel of the pilots' model; returns ((block, blocks/CU,
+        chunk), particle-steps/s).  Plain: one pilot; batch: pilots of one model."""
+        n = len(pilots)
+        arr = (abi.GsParams * n)(*pilots)
+        sh = abi.GsLaunchShape()
+        r = ctypes.c_double(0.0)
+        self._check(lib().gs_autotune_shape(self._ctx, arr, n, kernel, repeats, ctypes.byref(sh), ctypes.byref(r)),
+                    "gs_autotune_shape")
+        return sh.as_tuple(), r.value
+
+    def autotune_save(self, path):
+        self._check(lib().gs_autotune_save(self._ctx, path.encode()), "gs_autotune_save")
+
+    def autotune_load(self, path):
+        """Apply the cache's shapes for this device; returns how many."""
+        n = ctypes.c_uint32(0)
+        self._check(lib().gs_autotune_load(self._ctx, path.encode(), ctypes.byref(n)), "gs_autotune_load")
+        return n.value
 
     def set_order(self, mode):
         """ORDER_ENERGY (default) or ORDER_ID: the order in which plain runs

@@ -317,6 +317,14 @@ gs_ctx *context(int device) {
     gs_ctx *ctx = nullptr;
     const gs_status st = gs_create(&ctx, device);
     if (st != GS_OK) fail_exit(nullptr, st, "gs_create");
+    // launch shapes from the autotuner's cache, when one is named
+    if (const char *cache = std::getenv("GS_AUTOTUNE_CACHE")) {
+        uint32_t n = 0;
+        if (gs_autotune_load(ctx, cache, &n) != GS_OK)
+            std::fprintf(stderr, "[warning] autotune cache %s: %s\n", cache, gs_last_error(ctx));
+        else
+            log_info("Launch shapes from " + std::string(cache) + ": " + std::to_string(n) + " entries.");
+    }
     ctxs[device] = ctx;
     return ctx;
 }

@@ -243,9 +243,28 @@ gs_status gs_set_batch_budget(gs_ctx *ctx, uint64_t max_events);
  * OneDimensionFpModel.cu:172-184). */
 gs_status gs_set_plain_limits(gs_ctx *ctx, uint64_t max_chunk, uint64_t max_events);
 
-/* Override the launch shape used for `model` (0 = automatic). */
+/* Launch shape of a trajectory kernel.  block_threads: workgroup size (128,
+ * 256 or 512); blocks_per_cu: persistent blocks per CU, so that each lane
+ * integrates about N / (blocks_per_cu x CUs x block_threads) particles one
+ * after another (capped at the kernel's occupancy); chunk: particle ids a
+ * wave takes from the work queue per grab.  0 in a field: the default (256,
+ * the occupancy, 64).  Results never depend on the shape. */
+typedef struct gs_launch_shape {
+    uint32_t block_threads;
+    uint32_t blocks_per_cu;
+    uint32_t chunk;
+} gs_launch_shape;
+#define GS_KERNEL_PLAIN 0   /* gs_run_plain's trajectory kernel of a model   */
+#define GS_KERNEL_BATCH 1   /* gs_run_batch's trajectory kernel of a model   */
+
+/* Override the launch shape used for `model` by both kernels (0 = automatic);
+ * block_threads 0, 128, 256 or 512. */
 gs_status gs_set_launch(gs_ctx *ctx, int32_t model, uint32_t block_threads,
                         uint32_t blocks_per_cu);
+/* Set / read the launch shape of one kernel (GS_KERNEL_*) of `model`; the
+ * getter returns the shape a launch would use, defaults filled in. */
+gs_status gs_set_launch_shape(gs_ctx *ctx, int32_t model, int32_t kernel, const gs_launch_shape *shape);
+gs_status gs_get_launch_shape(const gs_ctx *ctx, int32_t model, int32_t kernel, gs_launch_shape *shape);
 
 /* Order in which gs_run_plain hands particles to the lanes (results do not
  * depend on it: particles are keyed by id and sums are exact).
@@ -259,13 +278,29 @@ gs_status gs_set_launch(gs_ctx *ctx, int32_t model, uint32_t block_threads,
 #define GS_ORDER_ID 0
 #define GS_ORDER_ENERGY 1
 gs_status gs_set_order(gs_ctx *ctx, int32_t mode);
-/* Pick blocks per CU for `model` by timing the pilot simulation at every
- * occupancy from 1 to the kernel's maximum (best of `repeats` runs each) and
- * apply it as gs_set_launch would.  best_bpcu / best_steps_per_s may be NULL.
- * Replaces the fixed 256-thread launch geometry of setThreadBlockSize
+/* Autotuner: pick the launch shape of one kernel of the pilots' model by
+ * timing the pilot work (GS_KERNEL_PLAIN: one simulation through
+ * gs_run_plain; GS_KERNEL_BATCH: n_pilots simulations of one model through
+ * gs_run_batch), best of `repeats` runs per shape, rated by particle-steps
+ * per second of trajectory-kernel time.  Search: blocks per CU from 1 to the
+ * occupancy at 256 threads; then 128 and 512 threads at about the same waves
+ * per CU and at their occupancy; then 16, 32, 128 and 256 ids per grab.  The
+ * best shape is applied (gs_set_launch_shape) and kept for gs_autotune_save.
+ * best_shape / best_steps_per_s may be NULL.  Replaces the fixed launch
+ * geometry chosen by compute capability in setThreadBlockSize
  * (reference CUDAKernel/src/OneDimensionBpGpuModel.cpp:45-92). */
+gs_status gs_autotune_shape(gs_ctx *ctx, const gs_params *pilots, uint32_t n_pilots, int32_t kernel,
+                            uint32_t repeats, gs_launch_shape *best_shape, double *best_steps_per_s);
+/* gs_autotune_shape of the plain kernel with one pilot; returns its blocks per CU. */
 gs_status gs_autotune(gs_ctx *ctx, const gs_params *pilot, uint32_t repeats,
                       uint32_t *best_bpcu, double *best_steps_per_s);
+/* The autotuner's cache, a text file with one line per device architecture,
+ * CU count, kernel and model ("<arch> <CUs> <plain|batch> <model>
+ * <block_threads> <blocks_per_cu> <chunk> <steps/s>").  save writes the
+ * shapes tuned on ctx, keeping the file's other lines; load applies the lines
+ * that match ctx's device (*n_applied may be NULL). */
+gs_status gs_autotune_save(const gs_ctx *ctx, const char *path);
+gs_status gs_autotune_load(gs_ctx *ctx, const char *path, uint32_t *n_applied);
 /* Statistics of the last run on ctx. */
 gs_status gs_last_run_stats(const gs_ctx *ctx, gs_run_stats *st);
 

@@ -132,7 +132,7 @@ def test_autotune_picks_a_valid_shape_and_keeps_results(engine):
         assert 1 <= bpcu <= 16 and rate > 0 and math.isfinite(rate)
         again, _, _ = engine.run_plain(p)
     finally:
-        engine.set_launch(abi.MODEL_1D_BP, 0, 0)
+        engine.set_launch_shape(abi.MODEL_1D_BP, abi.KERNEL_PLAIN)
     assert bytes(ref) == bytes(again)
 
 
@@ -209,3 +209,64 @@ def test_batch_records_equal_plain_records(engine):
         want = r[:caps[i]]
         assert len(recs[i]) == min(caps[i], n), i
         assert [bytes(a) for a in recs[i]] == [bytes(b) for b in want], i
+
+
+SHAPES = ((128, 0, 16), (512, 1, 256), (256, 2, 32), (128, 3, 128))
+
+
+@pytest.mark.parametrize("model", [abi.MODEL_1D_BP, abi.MODEL_1D_FP, abi.MODEL_2D_GELIOSPHERE])
+def test_results_independent_of_launch_shape(engine, model):
+    """Workgroup size, blocks per CU and ids per queue grab change which lane
+    integrates which particle and when, never a result: plain runs and
+    batches under every shape equal the default shape's bit for bit."""
+    n = {abi.MODEL_1D_BP: 200_000, abi.MODEL_1D_FP: 2_000_000, abi.MODEL_2D_GELIOSPHERE: 20_000}[model]
+    p = abi.make_params(model, N=n, seed=61, weights=ALL_WEIGHTS[:4])
+    sims = [p, abi.clone_params(p, seed=62, n_particles=n // 3, r_inj=7.0)]
+    ref, _, _ = engine.run_plain(p)
+    refb = engine.run_batch(sims)
+    for shape in SHAPES:
+        engine.set_launch_shape(model, abi.KERNEL_PLAIN, *shape)
+        engine.set_launch_shape(model, abi.KERNEL_BATCH, *shape)
+        try:
+            got, _, _ = engine.run_plain(p)
+            blk = engine.last_stats().block_threads
+            gotb = engine.run_batch(sims)
+            used = engine.launch_shape(model, abi.KERNEL_PLAIN)
+        finally:
+            engine.set_launch_shape(model, abi.KERNEL_PLAIN)
+            engine.set_launch_shape(model, abi.KERNEL_BATCH)
+        say("shape", shape, "-> used", used)
+        assert blk == shape[0] and used[0] == shape[0] and used[2] == shape[2]
+        assert bytes(got) == bytes(ref), shape
+        assert [bytes(x) for x in gotb] == [bytes(x) for x in refb], shape
+
+
+def test_autotuner_searches_shapes_and_caches_them(engine, tmp_path):
+    """The autotuner searches workgroup size, blocks per CU and ids per grab
+    for a model's plain and batch kernels, applies the best, and its cache
+    file carries the shapes to a new context on the same device."""
+    from geliosphere_amd.engine import Engine
+    p = abi.make_params(abi.MODEL_1D_BP, N=300_000, seed=71)
+    ref, _, _ = engine.run_plain(p)
+    pilots = [abi.make_params(abi.MODEL_1D_BP, N=20_000 + 5_000 * i, seed=72 + i, r_inj=1.0 + 10 * i)
+              for i in range(4)]
+    try:
+        sh, rate = engine.autotune_shape([p], abi.KERNEL_PLAIN, 1)
+        shb, rateb = engine.autotune_shape(pilots, abi.KERNEL_BATCH, 1)
+        say("tuned plain", sh, "%.3e" % rate, "batch", shb, "%.3e" % rateb)
+        assert sh[0] in (128, 256, 512) and sh[1] >= 1 and sh[2] in (16, 32, 64, 128, 256)
+        assert shb[0] in (128, 256, 512) and shb[1] >= 1 and rate > 0 and rateb > 0
+        assert engine.launch_shape(abi.MODEL_1D_BP, abi.KERNEL_PLAIN) == sh
+        again, _, _ = engine.run_plain(p)
+        assert bytes(again) == bytes(ref)
+        path = str(tmp_path / "autotune.txt")
+        engine.autotune_save(path)
+        lines = [l.split() for l in open(path) if not l.startswith("#")]
+        assert sorted(l[2] for l in lines) == ["batch", "plain"] and all(l[3] == "1" for l in lines)
+        with Engine(0) as e2:
+            assert e2.autotune_load(path) == 2
+            assert e2.launch_shape(abi.MODEL_1D_BP, abi.KERNEL_PLAIN) == sh
+            assert e2.launch_shape(abi.MODEL_1D_BP, abi.KERNEL_BATCH) == shb
+    finally:
+        engine.set_launch_shape(abi.MODEL_1D_BP, abi.KERNEL_PLAIN)
+        engine.set_launch_shape(abi.MODEL_1D_BP, abi.KERNEL_BATCH)
