@@ -371,28 +371,32 @@ __device__ void run_lanes(const DevParams *__restrict__ sims, uint32_t n_sims, c
                 }
             }
         } else {
-            // 2D Geliosphere: steps stay rolled (the step body is large); normals come from
-            // a four-entry buffer refilled from the next block when empty.  All
-            // lanes are block-aligned at the cycle start, so the refill branch
-            // is wave-uniform.
-            double b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;
-            int used = 4;
-            uint32_t nb = blk;
+            // 2D Geliosphere: steps stay rolled (the step body is large).
+            // Step s of the cycle takes normals 3s .. 3s+2: the carried tail
+            // of the previous block and the head of the next.  The split
+            // depends only on s (wave-uniform), so each phase is a scalar
+            // branch with its normals in fixed registers (no selects).
+            static_assert(NPS == 3 && kCycleSteps == 4, "three blocks serve the four steps of a cycle");
+            double c0 = 0.0, c1 = 0.0, c2 = 0.0;   // unused normals of the last block
 #pragma unroll 1
             for (int s = 0; s < kCycleSteps; ++s) {
-                double z[NPS];
-#pragma unroll
-                for (int i = 0; i < NPS; ++i) {
-                    if (used == 4) {
-                        double t[4];
-                        if constexpr (!kBatch) asm volatile("" : "+s"(k0), "+s"(k1));   // as above
-                        normal_block(pid, nb, k0, k1, bm, t);
-                        b0 = t[0]; b1 = t[1]; b2 = t[2]; b3 = t[3];
-                        ++nb;
-                        used = 0;
+                double z[3];
+                if (s == 3) {
+                    z[0] = c0; z[1] = c1; z[2] = c2;
+                } else {
+                    double t[4];
+                    if constexpr (!kBatch) asm volatile("" : "+s"(k0), "+s"(k1));   // as above
+                    normal_block(pid, blk + (uint32_t)s, k0, k1, bm, t);
+                    if (s == 0) {
+                        z[0] = t[0]; z[1] = t[1]; z[2] = t[2];
+                        c0 = t[3];
+                    } else if (s == 1) {
+                        z[0] = c0; z[1] = t[0]; z[2] = t[1];
+                        c0 = t[2]; c1 = t[3];
+                    } else {
+                        z[0] = c0; z[1] = c1; z[2] = t[0];
+                        c0 = t[1]; c1 = t[2]; c2 = t[3];
                     }
-                    z[i] = used == 0 ? b0 : used == 1 ? b1 : used == 2 ? b2 : b3;
-                    ++used;
                 }
                 if (active) {
                     if (steps >= cap) {
@@ -758,13 +762,13 @@ using namespace gs;
 #define GS_PLAIN_WAVES_BP 5   // 96 VGPRs, no spills: +3 % over the 98-VGPR, 4-wave shape
 #endif
 #ifndef GS_PLAIN_WAVES_GE
-#define GS_PLAIN_WAVES_GE 3   // 168 VGPRs, a few spills: +7 % over 2 waves at 186 VGPRs
+#define GS_PLAIN_WAVES_GE 3   // 154 VGPRs, no spills
 #endif
 #ifndef GS_PLAIN_WAVES_SP
 #define GS_PLAIN_WAVES_SP 1
 #endif
 #ifndef GS_BATCH_WAVES_BP
-#define GS_BATCH_WAVES_BP 5   // 96 VGPRs, no spills
+#define GS_BATCH_WAVES_BP 5   // 96 VGPRs (17 spilled)
 #endif
 constexpr int kBatchWaves1D = 1, kBatchWavesBp = GS_BATCH_WAVES_BP, kBatchWavesSp = 2, kBatchWavesGe = GS_BATCH_WAVES_GE;
 constexpr int kPlainWavesBp = GS_PLAIN_WAVES_BP;
