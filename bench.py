@@ -53,8 +53,20 @@ def parse():
     ap.add_argument("--batch-sims", type=int, default=256,
                     help="simulations in the batched leg (0: skip); 256 as in the batch benchmark")
     ap.add_argument("--batch-scale", type=float, default=0.01,
-                    help="particle counts of the batched leg relative to the batch sampler's 1e5-1e7")
+                    help="particle counts of the batched leg relative to the batch sampler's 1e5-1e7 "
+                         "(default 1 %%: the full-scale comparison is --batch-only --batch-scale 1)")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batched comparison (the batch benchmark at --batch-scale) and print it")
+    ap.add_argument("--batch-mode", choices=["both", "batched", "sequential"], default="both",
+                    help="with --batch-only: time gs_run_batch, sequential gs_run_plain, or both")
+    ap.add_argument("--batch-slice", default="",
+                    help="with --batch-only: simulations START:END of the sampled batch")
+    ap.add_argument("--batch-parts", type=int, default=1,
+                    help="with --batch-only: run the batched leg as this many consecutive gs_run_batch calls")
+    ap.add_argument("--hash-out", default="", help="with --batch-only: write per-simulation spectrum hashes here")
+    ap.add_argument("--autotune-cache", default=os.path.join(ROOT, "geliosphere_amd", "data", "autotune.txt"),
+                    help="launch shapes from the autotuner's cache (geliosphere_amd.autotune); '' for defaults")
     return ap.parse_args()
 
 
@@ -76,6 +88,18 @@ def calibrate_copy(mib):
     hip.hipFree(src)
     hip.hipFree(dst)
     print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n}), flush=True)
+
+
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(budget_s):
@@ -101,6 +125,7 @@ def cpu_baseline(budget_s):
             break
         n = int(n * max(1.5, min(4.0, (budget_s * 0.5) / max(dt, 1e-3))))
     return {"value": steps / t_total, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
             "sample": "1D B-p, %d particles (log-uniform 0.01-100 GeV, dt=500 s), %d steps in %.1f s"
                       % (sample_n, steps, t_total)}
 
@@ -128,15 +153,83 @@ def batch_leg(eng, n_sims, scale, seed=2024):
     t_seq = time.perf_counter() - t0
     steps = sum(o.total_steps for o in outs)
     return {"sims": n_sims, "particles": sum(p.n_particles for p in sims),
+            "scale": scale,
+            "note": "the batch benchmark at %g %% of its particle counts" % (100 * scale),
             "particle_count_range": [int(1e5 * scale), int(1e7 * scale)],
             "batched_spectra_per_s": n_sims / t_batch, "sequential_plain_spectra_per_s": n_sims / t_seq,
             "speedup": t_seq / t_batch, "batched_particle_steps_per_s": steps / t_batch,
             "lane_utilisation": st.active_cycles / max(st.lane_cycles, 1), "bit_identical_to_plain": same}
 
 
+def spectrum_hash(s):
+    import hashlib
+    return hashlib.sha256(bytes(s)).hexdigest()[:16]
+
+
+def batch_only(eng, a):
+    """The batch benchmark at --batch-scale (scale 1: the benchmark as
+    specified, 256 simulations at N 1e5-1e7): gs_run_batch over the sampled
+    simulations (in --batch-parts consecutive calls) and/or one gs_run_plain
+    per simulation, each leg timed; per-simulation spectrum hashes let
+    sessions that time different legs or slices check bit-identity."""
+    from geliosphere_amd import batch
+    sims = batch.sample_batch(a.batch_sims, seed=2024, n_scale=a.batch_scale)
+    lo, hi = 0, len(sims)
+    if a.batch_slice:
+        lo, hi = (int(v) for v in a.batch_slice.split(":"))
+    idx = list(range(lo, hi))
+    part = [sims[i] for i in idx]
+    res = {"sims": len(part), "slice": [lo, hi], "scale": a.batch_scale,
+           "particle_count_range": [int(1e5 * a.batch_scale), int(1e7 * a.batch_scale)],
+           "particles": sum(p.n_particles for p in part)}
+    hashes = {}
+    if a.batch_mode in ("both", "batched"):
+        k = max(1, a.batch_parts)
+        cuts = [len(part) * j // k for j in range(k + 1)]
+        t_batch, steps, act, lanes = 0.0, 0, 0, 0
+        for j in range(k):
+            chunk = part[cuts[j]:cuts[j + 1]]
+            t0 = time.perf_counter()
+            outs = eng.run_batch(chunk)
+            dt = time.perf_counter() - t0
+            st = eng.last_stats()
+            t_batch += dt
+            steps += sum(o.total_steps for o in outs)
+            act += st.active_cycles
+            lanes += st.lane_cycles
+            for i, o in zip(idx[cuts[j]:cuts[j + 1]], outs):
+                hashes.setdefault(i, {})["batched"] = spectrum_hash(o)
+            progress("batched part %d/%d: %d simulations in %.1f s" % (j + 1, k, len(chunk), dt))
+        res.update({"batched_seconds": t_batch, "batched_spectra_per_s": len(part) / t_batch,
+                    "batched_particle_steps_per_s": steps / t_batch, "batched_calls": k,
+                    "lane_utilisation": act / max(lanes, 1), "particle_steps": steps})
+    if a.batch_mode in ("both", "sequential"):
+        t_seq = 0.0
+        for n_done, (i, p) in enumerate(zip(idx, part)):
+            t0 = time.perf_counter()
+            s, _, _ = eng.run_plain(p)
+            dt = time.perf_counter() - t0
+            t_seq += dt
+            hashes.setdefault(i, {})["plain"] = spectrum_hash(s)
+            progress("plain %d/%d (simulation %d, %s, N=%d): %.2f s" % (n_done + 1, len(part), i,
+                                                                       abi_name(p.model), p.n_particles, dt))
+        res.update({"sequential_seconds": t_seq, "sequential_plain_spectra_per_s": len(part) / t_seq})
+    if "batched_seconds" in res and "sequential_seconds" in res:
+        res["speedup"] = res["sequential_seconds"] / res["batched_seconds"]
+        res["bit_identical_to_plain"] = all(h["batched"] == h["plain"] for h in hashes.values())
+    if a.hash_out:
+        with open(a.hash_out, "w") as f:
+            json.dump({str(i): h for i, h in sorted(hashes.items())}, f, indent=0)
+    return res
+
+
 def surrogate_leg(eng):
-    """Inference rate and held-out accuracy of the committed ensemble, and the
-    plain Monte Carlo rate at the training particle counts."""
+    """Inference rate and held-out accuracy of the committed ensemble, and
+    what answering every query costs: the ensemble serves the queries it
+    trusts, batched Monte Carlo (gs_run_batch, at the training particle
+    counts) the rest, and every 1D F-p query (the ensemble does not serve
+    that model).  Effective spectra/s = 1 / (1 / inference rate + fallback
+    share / Monte Carlo rate), per model and over the served models."""
     import numpy as np
     from geliosphere_amd import surrogate, surrogate_eval, surrogate_data
     data_dir = os.path.join(ROOT, "geliosphere_amd", "data")
@@ -150,21 +243,39 @@ def surrogate_leg(eng):
     acc = surrogate_eval.evaluate(sg, d)
     rate, dt = surrogate_eval.inference_rate(sg)
     n_train = dict(zip(surrogate_data.MODELS, np.load(dpath)["n_train"].tolist()))
+    n_eval = dict(zip(surrogate_data.MODELS, np.load(dpath)["n_eval"].tolist()))
     mc = {}
     for m in surrogate_data.MODELS:
-        sims = surrogate_data.sims_for(m, 2, int(n_train[m]), 99, 777)
+        n = int(n_train[m]) or int(surrogate_data.DEFAULT_N_TRAIN[m])
+        sims = surrogate_data.sims_for(m, 16, n, 99, 777)
+        eng.run_batch(sims[:4])
         t0 = time.perf_counter()
-        for p in sims:
-            eng.run_plain(p)
-        mc[abi_name(m)] = len(sims) / (time.perf_counter() - t0)
-    mc_rate = len(mc) / sum(1.0 / v for v in mc.values())
+        eng.run_batch(sims)
+        mc[abi_name(m)] = {"particles": n, "batched_spectra_per_s": len(sims) / (time.perf_counter() - t0)}
+    eff = {}
+    for m in surrogate_data.MODELS:
+        name = abi_name(m)
+        served = m in surrogate.SURROGATE_MODELS and name in acc
+        f = acc[name]["fallback_share"] if served else 1.0
+        eff[name] = 1.0 / (1.0 / rate + f / mc[name]["batched_spectra_per_s"])
+    served_names = [abi_name(m) for m in surrogate.SURROGATE_MODELS if abi_name(m) in acc]
+    f_all = float(np.mean([acc[k]["fallback_share"] for k in served_names])) if served_names else 1.0
+    mc_all = len(served_names) / sum(1.0 / mc[k]["batched_spectra_per_s"] for k in served_names)
     return {"inference_spectra_per_s": rate, "inference_batch": 65536, "inference_batch_ms": dt * 1e3,
-            "plain_mc_spectra_per_s_at_training_n": mc, "plain_mc_spectra_per_s_mean": mc_rate,
-            "speedup_vs_plain_mc": rate / mc_rate,
+            "served_models": served_names, "mc_only_models": [abi_name(m) for m in surrogate_data.MODELS
+                                                              if m not in surrogate.SURROGATE_MODELS],
+            "training_particles": {abi_name(m): int(n_train[m]) for m in surrogate_data.MODELS},
+            "evaluation_particles": {abi_name(m): int(n_eval[m]) for m in surrogate_data.MODELS},
+            "mc_fallback": mc,
+            "fallback_share": {k: v["fallback_share"] for k, v in acc.items()},
+            "effective_spectra_per_s": eff,
+            "effective_spectra_per_s_served": 1.0 / (1.0 / rate + f_all / mc_all),
+            "effective_speedup_vs_batched_mc_served": (1.0 / (1.0 / rate + f_all / mc_all)) / mc_all,
+            "surrogate_only_speedup_vs_batched_mc_served": rate / mc_all,
             "median_rel_err_0p1_10GeV": {k: v["median_rel_err_surrogate"] for k, v in acc.items()},
+            "median_rel_err_trusted_0p1_10GeV": {k: v["median_rel_err_trusted"] for k, v in acc.items()},
             "median_rel_err_nearest_neighbour": {k: v["median_rel_err_nearest_neighbour"] for k, v in acc.items()},
-            "mc_reference_median_rel_se": {k: v["median_rel_se_mc_reference"] for k, v in acc.items()},
-            "fallback_share": {k: v["fallback_share"] for k, v in acc.items()}}
+            "mc_reference_median_rel_se": {k: v["median_rel_se_mc_reference"] for k, v in acc.items()}}
 
 
 def abi_name(m):
@@ -186,6 +297,16 @@ def main():
 
     n = int(a.n)
     eng = Engine(0)
+    tuned = eng.autotune_load(a.autotune_cache) if a.autotune_cache and os.path.exists(a.autotune_cache) else 0
+    shapes = {abi.MODEL_NAMES[m]: {"plain": list(eng.launch_shape(m, abi.KERNEL_PLAIN)),
+                                   "batch": list(eng.launch_shape(m, abi.KERNEL_BATCH))} for m in range(4)}
+    shapes_info = {"cache": os.path.relpath(a.autotune_cache, ROOT) if tuned else None, "entries_applied": tuned,
+                   "shape": "[block_threads, blocks_per_cu, ids per queue grab]", "per_model": shapes}
+    if a.batch_only:
+        res = batch_only(eng, a)
+        eng.close()
+        print(json.dumps({"batch": res, "launch_shapes": shapes_info}), flush=True)
+        return
     models = {"bp+fp": (abi.MODEL_1D_BP, abi.MODEL_1D_FP), "bp": (abi.MODEL_1D_BP,),
               "fp": (abi.MODEL_1D_FP,), "sp": (abi.MODEL_2D_SOLARPROP,),
               "ge": (abi.MODEL_2D_GELIOSPHERE,)}[a.model]
@@ -249,6 +370,7 @@ def main():
             "lane_utilisation": {abi.MODEL_NAMES[m]: sum(r[4] for r in runs if r[0] == m) / a.steps
                                  for m in models},
             "longest_particle_steps": {abi.MODEL_NAMES[m]: max(r[5] for r in runs if r[0] == m) for m in models},
+            "launch_shapes": shapes_info,
         },
     }
     progress("plain MC legs done: %.4g particle-steps/s" % out["value"])

@@ -7,12 +7,13 @@ cache (gs_autotune_save), which the engine applies with gs_autotune_load.
 
 Plain kernels are tuned on one simulation of the model (the benchmark's
 parameters: K0 = 5e22 cm^2/s, V = 400 km/s, log-uniform 0.01-100 GeV) large
-enough that throughput, not the longest trajectory, sets its time (1D B-p
-2e7 particles, 2D 2e6; 1D F-p 1e8, whose runs are always set by their longest
-trajectory); batch kernels on 16 simulations of the model drawn by the batch
-benchmark's sampler at 10 % of its particle counts.  A pilot of about one
-particle per lane would tune for the tail instead.  The default cache is
-geliosphere_amd/data/autotune.txt.
+enough that throughput, not the longest trajectory, sets its time: 1D B-p
+2e7 particles, 2D 1e7 (the 2D sweep's size; at 2e6, about ten particles per
+lane, the longest trajectories still set a third of the time and the search
+then favours fewer lanes), 1D F-p 1e8 (its runs are always set by their
+longest trajectory).  Batch kernels are tuned on 16 simulations of the model
+drawn by the batch benchmark's sampler at 10 % of its particle counts.  The
+default cache is geliosphere_amd/data/autotune.txt.
 """
 import argparse
 import os
@@ -26,7 +27,7 @@ from .engine import Engine
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_CACHE = os.path.join(HERE, "data", "autotune.txt")
 PLAIN_N = {abi.MODEL_1D_FP: 100_000_000, abi.MODEL_1D_BP: 20_000_000,
-           abi.MODEL_2D_SOLARPROP: 2_000_000, abi.MODEL_2D_GELIOSPHERE: 2_000_000}
+           abi.MODEL_2D_SOLARPROP: 10_000_000, abi.MODEL_2D_GELIOSPHERE: 10_000_000}
 BATCH_SCALE = 0.1
 
 
