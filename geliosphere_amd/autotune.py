@@ -42,6 +42,7 @@ def main(argv=None):
     ap.add_argument("--repeats", type=int, default=2)
     ap.add_argument("--models", default="0,1,2,3")
     a = ap.parse_args(argv)
+    os.environ.setdefault("GS_AUTOTUNE_LOG", "1")   # a line per measured shape (stderr)
     eng = Engine(0)
     for m in [int(v) for v in a.models.split(",")]:
         t0 = time.time()

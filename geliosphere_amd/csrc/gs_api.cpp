@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -1028,10 +1029,16 @@ gs_status gs_autotune_shape(gs_ctx *ctx, const gs_params *pilots, uint32_t n_pil
     };
     gs_launch_shape best{256, 0, (uint32_t)gs::kChunk};
     double best_rate = -1.0;
+    // GS_AUTOTUNE_LOG set: one line per measured shape on stderr
+    const char *log_env = std::getenv("GS_AUTOTUNE_LOG");
+    const bool log = log_env && *log_env && std::strcmp(log_env, "0") != 0;
     auto consider = [&](const gs_launch_shape &sh) -> gs_status {
         double v = 0.0;
         const gs_status st = measure(sh, &v);
         if (st == GS_OK && v > best_rate) { best_rate = v; best = sh; }
+        if (log)
+            std::fprintf(stderr, "gs_autotune: model %d %s: %u threads, %u blocks/CU, grab %u: %.4g particle-steps/s\n",
+                         model, batch ? "batch" : "plain", sh.block_threads, sh.blocks_per_cu, sh.chunk, v);
         return st;
     };
     gs_status st = GS_OK;

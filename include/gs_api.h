@@ -286,7 +286,9 @@ gs_status gs_set_order(gs_ctx *ctx, int32_t mode);
  * occupancy at 256 threads; then 128 and 512 threads at about the same waves
  * per CU and at their occupancy; then 16, 32, 128 and 256 ids per grab.  The
  * best shape is applied (gs_set_launch_shape) and kept for gs_autotune_save.
- * best_shape / best_steps_per_s may be NULL.  Replaces the fixed launch
+ * best_shape / best_steps_per_s may be NULL.  With the environment variable
+ * GS_AUTOTUNE_LOG set (not "0"), each measured shape and its rate is printed
+ * on stderr as it completes.  Replaces the fixed launch
  * geometry chosen by compute capability in setThreadBlockSize
  * (reference CUDAKernel/src/OneDimensionBpGpuModel.cpp:45-92). */
 gs_status gs_autotune_shape(gs_ctx *ctx, const gs_params *pilots, uint32_t n_pilots, int32_t kernel,
