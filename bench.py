@@ -241,7 +241,8 @@ def surrogate_leg(eng):
     sg.load(wpath)
     d = surrogate_eval.load([dpath])
     acc = surrogate_eval.evaluate(sg, d)
-    rate, dt = surrogate_eval.inference_rate(sg)
+    rate_host, dt_host = surrogate_eval.inference_rate(sg)
+    rate, dt = surrogate_eval.inference_rate_device(sg)
     n_train = dict(zip(surrogate_data.MODELS, np.load(dpath)["n_train"].tolist()))
     n_eval = dict(zip(surrogate_data.MODELS, np.load(dpath)["n_eval"].tolist()))
     mc = {}
@@ -262,6 +263,8 @@ def surrogate_leg(eng):
     f_all = float(np.mean([acc[k]["fallback_share"] for k in served_names])) if served_names else 1.0
     mc_all = len(served_names) / sum(1.0 / mc[k]["batched_spectra_per_s"] for k in served_names)
     return {"inference_spectra_per_s": rate, "inference_batch": 65536, "inference_batch_ms": dt * 1e3,
+            "inference_buffers": "device-resident (gs_surrogate_predict_device)",
+            "inference_spectra_per_s_host_buffers": rate_host, "inference_batch_ms_host_buffers": dt_host * 1e3,
             "served_models": served_names, "mc_only_models": [abi_name(m) for m in surrogate_data.MODELS
                                                               if m not in surrogate.SURROGATE_MODELS],
             "training_particles": {abi_name(m): int(n_train[m]) for m in surrogate_data.MODELS},

@@ -227,14 +227,16 @@ gs_status backward(gs_surrogate *s, const __bf16 *X, size_t xstride, int rows, c
     return GS_OK;
 }
 
-// fp32 rows [n][n_inputs] -> standardised, padded bf16 rows [n][kPad] on the device.
-gs_status upload_inputs(gs_surrogate *s, const float *x, uint32_t n, __bf16 **out) {
+// fp32 rows [n][n_inputs] (host memory, or device memory if x_on_device) ->
+// standardised, padded bf16 rows [n][kPad] on the device.
+gs_status upload_inputs(gs_surrogate *s, const float *x, uint32_t n, __bf16 **out, bool x_on_device = false) {
     const uint32_t ni = s->cfg.n_inputs;
     float *xf;
     __bf16 *xb;
     SG_NEED(s, B_XF, (size_t)n * ni, xf);
     SG_NEED(s, B_X, (size_t)n * kPad, xb);
-    SG_TRY(s, hipMemcpyAsync(xf, x, (size_t)n * ni * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    SG_TRY(s, hipMemcpyAsync(xf, x, (size_t)n * ni * sizeof(float),
+                             x_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s->stream));
     SG_TRY(s, sg_launch_standardise(xf, (int)n, (int)ni, kPad, s->mu, s->sd, xb, s->stream));
     SG_TRY(s, hipStreamSynchronize(s->stream));
     *out = xb;
@@ -578,8 +580,11 @@ gs_status gs_surrogate_gradients(gs_surrogate *s, const float *x, const float *y
     return GS_OK;
 }
 
-gs_status gs_surrogate_predict(gs_surrogate *s, const float *x, uint32_t n, float *mean, float *sd,
-                               uint8_t *fallback) {
+// Both predict entry points: host buffers (copied through the surrogate's
+// stream) or device buffers (the statistics and flags kernels write into the
+// caller's rows directly).
+static gs_status predict_rows(gs_surrogate *s, const float *x, uint32_t n, float *mean, float *sd,
+                              uint8_t *fallback, bool on_device) {
     if (!s) return GS_ERR_INVALID_ARG;
     if (!x && n) return fail(s, GS_ERR_INVALID_ARG, "NULL inputs");
     if (n == 0) return GS_OK;
@@ -591,31 +596,46 @@ gs_status gs_surrogate_predict(gs_surrogate *s, const float *x, uint32_t n, floa
         const uint32_t rows = std::min(chunk, n - r0);
         __bf16 *X;
         float *out, *dm, *ds;
-        gs_status st = upload_inputs(s, x + (size_t)r0 * ni, rows, &X);
+        gs_status st = upload_inputs(s, x + (size_t)r0 * ni, rows, &X, on_device);
         if (st != GS_OK) return st;
         SG_NEED(s, B_OUT, (size_t)E * rows * kPad, out);
-        SG_NEED(s, B_MEAN, (size_t)rows * no, dm);
-        SG_NEED(s, B_SD, (size_t)rows * no, ds);
+        if (on_device && mean) dm = mean + (size_t)r0 * no;
+        else SG_NEED(s, B_MEAN, (size_t)rows * no, dm);
+        if (on_device && sd) ds = sd + (size_t)r0 * no;
+        else SG_NEED(s, B_SD, (size_t)rows * no, ds);
         st = forward_infer(s, X, (int)rows, out);
         if (st != GS_OK) return st;
         SG_TRY(s, sg_launch_stats(out, E, (int)rows, kPad, (int)no, (size_t)rows * kPad, dm, ds, s->stream));
-        // Flags on the device (from the raw inputs still in B_XF); results go
-        // straight into the caller's rows.
+        // Flags on the device (from the raw inputs still in B_XF).
         const size_t bytes = (size_t)rows * no * sizeof(float);
         if (fallback) {
             const float *xraw;
             uint8_t *fl;
             SG_NEED(s, B_XF, (size_t)rows * ni, xraw);
-            SG_NEED(s, B_FLAGS, (size_t)rows, fl);
+            if (on_device) fl = fallback + r0;
+            else SG_NEED(s, B_FLAGS, (size_t)rows, fl);
             SG_TRY(s, sg_launch_flags(ds, xraw, (int)rows, (int)no, (int)ni, s->cfg.std_outputs, s->cfg.std_limit,
                                       s->lo, s->hi, fl, s->stream));
-            SG_TRY(s, hipMemcpyAsync(fallback + r0, fl, rows, hipMemcpyDeviceToHost, s->stream));
+            if (!on_device) SG_TRY(s, hipMemcpyAsync(fallback + r0, fl, rows, hipMemcpyDeviceToHost, s->stream));
         }
-        if (mean) SG_TRY(s, hipMemcpyAsync(mean + (size_t)r0 * no, dm, bytes, hipMemcpyDeviceToHost, s->stream));
-        if (sd) SG_TRY(s, hipMemcpyAsync(sd + (size_t)r0 * no, ds, bytes, hipMemcpyDeviceToHost, s->stream));
+        if (!on_device) {
+            if (mean) SG_TRY(s, hipMemcpyAsync(mean + (size_t)r0 * no, dm, bytes, hipMemcpyDeviceToHost, s->stream));
+            if (sd) SG_TRY(s, hipMemcpyAsync(sd + (size_t)r0 * no, ds, bytes, hipMemcpyDeviceToHost, s->stream));
+        }
+        // B_XF and the scratch outputs are reused by the next chunk
         SG_TRY(s, hipStreamSynchronize(s->stream));
     }
     return GS_OK;
+}
+
+gs_status gs_surrogate_predict(gs_surrogate *s, const float *x, uint32_t n, float *mean, float *sd,
+                               uint8_t *fallback) {
+    return predict_rows(s, x, n, mean, sd, fallback, false);
+}
+
+gs_status gs_surrogate_predict_device(gs_surrogate *s, const float *x, uint32_t n, float *mean, float *sd,
+                                      uint8_t *fallback) {
+    return predict_rows(s, x, n, mean, sd, fallback, true);
 }
 
 }  // extern "C"

@@ -53,13 +53,14 @@ def lib():
         L.gs_surrogate_last_error.restype = ctypes.c_char_p
         L.gs_surrogate_train.argtypes = [vp, fp, fp, fp, ctypes.c_uint32, ctypes.c_uint32, fp]
         L.gs_surrogate_predict.argtypes = [vp, fp, ctypes.c_uint32, fp, fp, P(ctypes.c_uint8)]
+        L.gs_surrogate_predict_device.argtypes = [vp, vp, ctypes.c_uint32, vp, vp, vp]
         L.gs_surrogate_gradients.argtypes = [vp, fp, fp, fp, ctypes.c_uint32, fp, fp]
         L.gs_surrogate_layout.argtypes = [vp, P(ctypes.c_uint64), P(ctypes.c_uint64), ctypes.c_uint32]
         L.gs_surrogate_get_params.argtypes = [vp, fp, ctypes.c_uint64]
         L.gs_surrogate_set_params.argtypes = [vp, fp, ctypes.c_uint64]
         L.gs_surrogate_get_normalization.argtypes = [vp, fp, fp, fp, fp]
         L.gs_surrogate_set_normalization.argtypes = [vp, fp, fp, fp, fp]
-        for f in ("default_config", "create", "train", "predict", "gradients", "layout", "get_params",
+        for f in ("default_config", "create", "train", "predict", "predict_device", "gradients", "layout", "get_params",
                   "set_params", "get_normalization", "set_normalization"):
             getattr(L, "gs_surrogate_" + f).restype = ctypes.c_int
         _bound = True
@@ -175,6 +176,26 @@ class Surrogate:
         self._check(lib().gs_surrogate_predict(self._h, px, n, mean.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
                                                psd, fb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))), "predict")
         return mean, sd, fb
+
+    def predict_device(self, x, mean=None, sd=None, fallback=None):
+        """gs_surrogate_predict_device on device buffers (devmem.DeviceArray)
+        of the surrogate's device: x float32 [n, n_inputs]; mean, sd float32
+        [n, n_outputs] and fallback uint8 [n], allocated here when not given
+        (sd only on request: pass a buffer).  Returns (mean, sd, fallback)."""
+        from .devmem import DeviceArray
+        n, no = x.shape[0], self.cfg.n_outputs
+        if x.dtype != np.float32 or x.shape != (n, self.cfg.n_inputs):
+            raise ValueError("x must be a float32 [n, %d] device array" % self.cfg.n_inputs)
+        if mean is None:
+            mean = DeviceArray((n, no), np.float32)
+        if fallback is None:
+            fallback = DeviceArray((n,), np.uint8)
+        for t, shape, dt in ((mean, (n, no), np.float32), (sd, (n, no), np.float32), (fallback, (n,), np.uint8)):
+            if t is not None and (t.shape != shape or t.dtype != dt):
+                raise ValueError("output buffer of shape %s, dtype %s expected" % (shape, np.dtype(dt)))
+        self._check(lib().gs_surrogate_predict_device(self._h, x.ptr, n, mean.ptr, sd.ptr if sd is not None else None,
+                                                      fallback.ptr), "predict_device")
+        return mean, sd, fallback
 
     def save(self, path):
         mean, sd, lo, hi = self.get_normalization()

@@ -10,9 +10,14 @@ model fixed per set and the particle count per model set by the caller.
         [--train-per-model 1000] [--eval-per-model 40]
         [--n-train 1D_FP,1D_BP,2D_SP,2D_GE] [--n-eval ...] [--seed S]
         [--models 0,1,2,3]
+    python -m geliosphere_amd.surrogate_data --out FILE.npz --join A.npz B.npz ...
 
-writes x, y, w (training), ex, ey, ew (evaluation), the model of every row
-and the particle counts used.
+writes x, y, w (training), ex, ey, ew (evaluation), the model of every row,
+the particle counts used, and `sets`: one row per model generated, [model,
+training spectra, training particles, evaluation spectra, evaluation
+particles, seed], from which sims_for() re-draws every simulation (training
+seed S, evaluation seed S + 1).  Files made in several runs are joined with
+merge().
 """
 import argparse
 import random
@@ -54,6 +59,26 @@ def run_set(engine, sims, label, chunk=32):
     return np.concatenate(xs), np.concatenate(ys), np.concatenate(ws)
 
 
+def redraw(sets):
+    """(training simulations, evaluation simulations) of a data set's `sets`
+    rows, in the order of its x and ex rows."""
+    tr, ev = [], []
+    for m, n_tr, p_tr, n_ev, p_ev, seed in np.asarray(sets).tolist():
+        tr += sims_for(m, n_tr, p_tr, seed, 1_000_000 * (m + 1))
+        ev += sims_for(m, n_ev, p_ev, seed + 1, 50_000_000 * (m + 1))
+    return tr, ev
+
+
+def merge(paths, out):
+    """Join data sets made by separate runs (rows in the order given)."""
+    ds = [np.load(p, allow_pickle=False) for p in paths]
+    res = {k: np.concatenate([d[k] for d in ds]) for k in ("x", "y", "w", "m", "ex", "ey", "ew", "em", "sets")}
+    res["n_train"] = np.max(np.stack([d["n_train"] for d in ds]), axis=0)
+    res["n_eval"] = np.max(np.stack([d["n_eval"] for d in ds]), axis=0)
+    np.savez_compressed(out, **res)
+    return res
+
+
 def parse_counts(s, default):
     if not s:
         return dict(default)
@@ -70,7 +95,13 @@ def main(argv=None):
     ap.add_argument("--n-eval", default="")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--models", default="0,1,2,3", help="gs_model values to generate")
+    ap.add_argument("--join", nargs="+", default=None, metavar="NPZ",
+                    help="write the join of these data sets to --out (no generation)")
     a = ap.parse_args(argv)
+    if a.join:
+        res = merge(a.join, a.out)
+        print("wrote %s: %d training, %d evaluation spectra" % (a.out, len(res["x"]), len(res["ex"])), flush=True)
+        return 0
     models = [int(v) for v in a.models.split(",")]
     n_train = parse_counts(a.n_train, DEFAULT_N_TRAIN)
     n_eval = parse_counts(a.n_eval, DEFAULT_N_EVAL)
@@ -86,6 +117,8 @@ def main(argv=None):
     out = {k: np.concatenate(v) for k, v in parts.items()}
     out["n_train"] = np.array([n_train[m] if m in models else 0 for m in MODELS])
     out["n_eval"] = np.array([n_eval[m] if m in models else 0 for m in MODELS])
+    out["sets"] = np.array([[m, a.train_per_model, n_train[m], a.eval_per_model, n_eval[m], a.seed] for m in models],
+                           dtype=np.int64)
     np.savez_compressed(a.out, **out)
     print("wrote %s: %d training, %d evaluation spectra" % (a.out, len(out["x"]), len(out["ex"])), flush=True)
     eng.close()

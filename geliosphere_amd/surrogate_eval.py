@@ -86,6 +86,7 @@ def evaluate(sg, d):
 
 
 def inference_rate(sg, n=65536, repeats=5):
+    """(queries/s, seconds per call) of gs_surrogate_predict on host arrays."""
     rng = np.random.default_rng(0)
     lo, hi = sg.get_normalization()[2:]
     x = (lo + (hi - lo) * rng.uniform(size=(n, len(lo)))).astype(np.float32)
@@ -94,6 +95,26 @@ def inference_rate(sg, n=65536, repeats=5):
     for _ in range(repeats):
         sg.predict(x)
     dt = (time.perf_counter() - t0) / repeats
+    return n / dt, dt
+
+
+def inference_rate_device(sg, n=65536, repeats=5):
+    """(queries/s, seconds per call) of gs_surrogate_predict_device on
+    device-resident inputs and outputs (mean and flags, as a serving loop
+    reads them)."""
+    from .devmem import DeviceArray
+    rng = np.random.default_rng(0)
+    lo, hi = sg.get_normalization()[2:]
+    x = DeviceArray.from_host((lo + (hi - lo) * rng.uniform(size=(n, len(lo)))).astype(np.float32))
+    mean = DeviceArray((n, surrogate.N_OUT), np.float32)
+    fb = DeviceArray((n,), np.uint8)
+    sg.predict_device(x, mean, None, fb)   # warm
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        sg.predict_device(x, mean, None, fb)
+    dt = (time.perf_counter() - t0) / repeats
+    for b in (x, mean, fb):
+        b.free()
     return n / dt, dt
 
 

@@ -69,6 +69,14 @@ gs_status gs_surrogate_train(gs_surrogate *s, const float *x, const float *y, co
 gs_status gs_surrogate_predict(gs_surrogate *s, const float *x, uint32_t n, float *mean, float *sd,
                                uint8_t *fallback);
 
+/* gs_surrogate_predict with every buffer in device memory of the
+ * surrogate's device (x: n x n_inputs floats; mean, sd: n x n_outputs; fallback:
+ * n bytes; NULL allowed for the outputs): no host copies.  The inputs must be
+ * complete when the call is made (it runs on the surrogate's own stream);
+ * the outputs are complete on return. */
+gs_status gs_surrogate_predict_device(gs_surrogate *s, const float *x, uint32_t n, float *mean, float *sd,
+                                      uint8_t *fallback);
+
 /* One forward and backward pass on the given rows for every member, no
  * update: grads receives d loss / d params in the layout of
  * gs_surrogate_get_params, loss one value per member. */

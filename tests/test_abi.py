@@ -12,8 +12,8 @@ from geliosphere_amd import abi, engine
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
-    text = open(os.path.join(ROOT, "include", "gs_api.h")).read()
+def declared_functions(header="gs_api.h"):
+    text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(gs_[a-z_]+)\s*\(", text)))
 
@@ -22,6 +22,14 @@ def test_library_exports_every_declared_symbol():
     L = engine.lib()
     names = declared_functions()
     assert set(names) == set(engine.EXPORTS)
+    for n in names:
+        assert hasattr(L, n), n
+
+
+def test_library_exports_every_surrogate_symbol():
+    L = engine.lib()
+    names = declared_functions("gs_surrogate.h")
+    assert "gs_surrogate_predict_device" in names
     for n in names:
         assert hasattr(L, n), n
 

@@ -157,3 +157,29 @@ def test_fallback_flags_and_save_load(tmp_path):
     want = np.array([(0x1 if any((mask >> c) & 1 and not sd4[r, c] <= u.cfg.std_limit for c in range(30)) else 0) |
                      (0x2 if not np.all((q[r] >= lo) & (q[r] <= hi)) else 0) for r in range(len(q))], dtype=np.uint8)
     assert np.array_equal(fb4, want) and 0 < np.count_nonzero(fb4 & 0x1) < len(q)
+
+
+def test_predict_device_equals_host_predict():
+    """gs_surrogate_predict_device (device buffers, no host copies) returns
+    the host entry point's means, spreads and flags bit for bit, over more
+    rows than one 65,536-row chunk and with optional outputs left out."""
+    from geliosphere_amd.devmem import DeviceArray
+    rng = np.random.default_rng(16)
+    x, y, w = synthetic(2048, rng)
+    s = make(seed=5, batch=256)
+    s.train(x, y, w, 2)
+    sd_all = s.predict(x)[1]
+    u = make(seed=5, batch=256, std_limit=float(np.median(sd_all.max(axis=1))))
+    u.set_params(s.get_params())
+    u.set_normalization(*s.get_normalization())
+    q = np.vstack([rng.uniform(-1.2, 1.2, (70_000, NI)).astype(np.float32), np.full((1, NI), 5.0, np.float32)])
+    m_h, sd_h, fb_h = u.predict(q)
+    xd = DeviceArray.from_host(q)
+    sd_d = DeviceArray((len(q), 30), np.float32)
+    m_d, sd_d, fb_d = u.predict_device(xd, None, sd_d, None)
+    assert np.array_equal(m_d.to_host(), m_h) and np.array_equal(sd_d.to_host(), sd_h)
+    assert np.array_equal(fb_d.to_host(), fb_h)
+    assert 0 < np.count_nonzero(fb_h & 0x1) < len(q) and fb_h[-1] & 0x2
+    x100 = DeviceArray.from_host(q[:100])
+    m2, _, fb2 = u.predict_device(x100)
+    assert np.array_equal(m2.to_host(), m_h[:100]) and np.array_equal(fb2.to_host(), fb_h[:100])
