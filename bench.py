@@ -31,7 +31,7 @@ FP64_PEAK_TFLOPS = 78.6          # MI355X dense FP64 vector
 # WRITE_SIZE x 1.0) KiB per launch, scaled by a 256 MiB copy, over 1e8
 # particles (profiles/plain_bp_hbm_traffic.md).  Algorithmic: one 64-byte
 # event record per particle.
-BP_TRAFFIC_BYTES_PER_PARTICLE = 79.2
+BP_TRAFFIC_BYTES_PER_PARTICLE = 83.4
 # Algorithmic FLOPs per particle-step of the reference formulation, counted
 # by the oracle's op-counting scalar (tests/test_oracle.py::test_flop_count_per_step).
 ALG_FLOPS_PER_STEP = {0: 42.0, 1: 40.0, 2: 164.0, 3: 258.6}
