@@ -385,7 +385,13 @@ def main():
         progress("surrogate leg done")
     eng.close()
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
+        cb = cpu_baseline(a.cpu_seconds)
+        # spectra/s of each side for the step's models at the step's N: the
+        # steps one spectrum takes (measured above) over each side's rate
+        steps_per_step = sum(r[1] for r in runs) / a.steps
+        cb["spectra_per_s_at_step_size"] = len(models) * cb["value"] / steps_per_step
+        cb["gpu_spectra_per_s_at_step_size"] = len(models) * a.steps / wall
+        out["cpu_baseline"] = cb
         progress("CPU baseline done")
     print(json.dumps(out), flush=True)
 

@@ -34,3 +34,29 @@ def test_weights_match_the_input_width():
     w = np.load(os.path.join(ROOT, "geliosphere_amd", "data", "surrogate_weights.npz"), allow_pickle=False)
     assert w["mean"].shape == (surrogate.N_IN,) and w["lo"].shape == (surrogate.N_IN,)
     assert np.all(w["sd"] > 0) and np.all(w["lo"] <= w["hi"])
+
+
+def test_merge_joins_rows_and_keeps_provenance(tmp_path):
+    """merge() concatenates every row array in order and keeps each part's
+    `sets` rows; per-model particle counts are the parts' (one part per model)."""
+    from geliosphere_amd import surrogate_data as sd
+    parts = []
+    for k, m in enumerate((1, 3)):
+        n, e = 3 + k, 2
+        d = {"x": np.full((n, 8), m, np.float32), "y": np.zeros((n, 30), np.float32),
+             "w": np.ones((n, 30), np.float32), "m": np.full(n, m), "ex": np.full((e, 8), -m, np.float32),
+             "ey": np.zeros((e, 30), np.float32), "ew": np.ones((e, 30), np.float32), "em": np.full(e, m),
+             "n_train": np.array([0, 1000 if m == 1 else 0, 0, 500 if m == 3 else 0]),
+             "n_eval": np.array([0, 10000 if m == 1 else 0, 0, 5000 if m == 3 else 0]),
+             "sets": np.array([[m, n, 1000 if m == 1 else 500, e, 10000 if m == 1 else 5000, 31]])}
+        p = tmp_path / ("part%d.npz" % k)
+        np.savez(p, **d)
+        parts.append(str(p))
+    out = tmp_path / "joined.npz"
+    res = sd.merge(parts, str(out))
+    j = np.load(out, allow_pickle=False)
+    assert j["x"].shape == (7, 8) and list(j["m"]) == [1, 1, 1, 3, 3, 3, 3]
+    assert list(j["em"]) == [1, 1, 3, 3] and np.all(j["ex"][:2] == -1)
+    assert list(j["n_train"]) == [0, 1000, 0, 500] and list(j["n_eval"]) == [0, 10000, 0, 5000]
+    assert j["sets"].tolist() == [[1, 3, 1000, 2, 10000, 31], [3, 4, 500, 2, 5000, 31]]
+    assert np.array_equal(res["x"], j["x"])
