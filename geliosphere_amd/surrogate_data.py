@@ -47,6 +47,10 @@ def sims_for(model, count, n_particles, seed, first_sim_seed):
 
 
 def run_set(engine, sims, label, chunk=32):
+    if not sims:   # an evaluation-only or training-only run
+        n_out = surrogate.N_OUT
+        return (np.zeros((0, surrogate.N_IN), np.float32), np.zeros((0, n_out), np.float32),
+                np.zeros((0, n_out), np.float32))
     xs, ys, ws = [], [], []
     t0 = time.time()
     for i in range(0, len(sims), chunk):
