@@ -22,6 +22,8 @@
 //   * noise: Philox4x32-10 keyed by the seed, counter (id, block, purpose),
 //     Box-Muller in double, instead of std::mt19937 +
 //     std::normal_distribution<float> (OneDimensionBpCpuModel.cpp:60-62);
+//     the flag ORACLE_FLAG_FAITHFUL restores the latter, with the
+//     reference's per-iteration thread schedule, for timing the CPU path;
 //   * beta in full double precision (the reference takes sqrtf,
 //     OneDimensionBpCpuModel.cpp:79);
 //   * 2D Geliosphere draws 3 normals per step: the reference's B13 and B23
@@ -38,6 +40,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <random>
 #include <thread>
 #include <vector>
 
@@ -133,6 +136,12 @@ inline void philox10(const uint32_t in[4], const uint32_t kin[2], uint32_t out[4
 
 const double kTwoM32 = 2.3283064365386963e-10;  // 2^-32
 
+// Timing mode with the reference's random source (ORACLE_FLAG_FAITHFUL): the
+// running thread's std::mt19937 and std::normal_distribution<float>
+// (OneDimensionBpCpuModel.cpp:60-62), set by oracle_run's worker.
+thread_local std::mt19937 *g_faithful_gen = nullptr;
+thread_local std::normal_distribution<float> *g_faithful_dist = nullptr;
+
 // Normal stream of one particle: block k = Philox(ctr = {id lo, id hi, k, 0},
 // key = {seed lo, seed hi}) gives normals 4k..4k+3 as two Box-Muller pairs
 // (x0,x1) and (x2,x3): u1 = (x + 1) 2^-32 in (0,1], u2 = x' 2^-32 in [0,1),
@@ -163,6 +172,7 @@ struct NormalStream {
         idx = 0;
     }
     double next() {
+        if (g_faithful_gen) return (double)(*g_faithful_dist)(*g_faithful_gen) * scale;
         if (idx == 4) refill();
         return buf[idx++] * scale;
     }
@@ -908,9 +918,49 @@ int oracle_run(const gs_params *p, gs_spectrum *out, gs_exit_record *recs, uint6
             }
         }
     };
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; ++t) th.emplace_back(worker, t);
-    for (auto &x : th) x.join();
+    if (P.flags & ORACLE_FLAG_FAITHFUL) {
+        // The reference's schedule (OneDimensionBpCpuModel.cpp:27-41,63-66):
+        // per iteration, hardware_concurrency() threads are spawned and
+        // joined, thread t taking the 101 x 250 particles from
+        // (iteration * threads + t) * 25250; each new thread seeds its own
+        // generator.
+        const uint64_t per = 101 * 250;
+        for (int t = 0; t < nt; ++t) hists[t] = new Hist();
+        for (uint64_t it = 0; it * per * (uint64_t)nt < N; ++it) {
+            std::vector<std::thread> th;
+            for (int t = 0; t < nt; ++t)
+                th.emplace_back([&, t, it]() {
+                    std::seed_seq seq{(uint32_t)P.seed, (uint32_t)(P.seed >> 32), (uint32_t)it, (uint32_t)t};
+                    std::mt19937 gen(seq);
+                    std::normal_distribution<float> dist(0.0f, 1.0f);
+                    g_faithful_gen = &gen;
+                    g_faithful_dist = &dist;
+                    Hist *h = hists[t];
+                    std::vector<gs_exit_record> local;
+                    Emitter em{&local};
+                    const uint64_t s0 = (it * (uint64_t)nt + (uint64_t)t) * per;
+                    for (uint64_t i = s0; i < std::min(N, s0 + per); ++i) {
+                        local.clear();
+                        ParticleEnd end = run_particle<double>(P, P.first_particle + i, em);
+                        h->n_particles++;
+                        h->total_steps += end.steps;
+                        if (end.status == 1) h->n_nan++;
+                        if (end.status == 2) h->n_step_cap++;
+                        for (const auto &ev : local) {
+                            add_event(P, *h, ev);
+                            if (keep) evs[t].push_back(ev);
+                        }
+                    }
+                    g_faithful_gen = nullptr;
+                    g_faithful_dist = nullptr;
+                });
+            for (auto &x : th) x.join();
+        }
+    } else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t) th.emplace_back(worker, t);
+        for (auto &x : th) x.join();
+    }
     Hist *tot = new Hist();
     for (int t = 0; t < nt; ++t) { merge_hist(*tot, *hists[t]); delete hists[t]; }
 

@@ -23,6 +23,12 @@
 /* gs_params.flags bit read by the oracle only: every normal is 0, so a step
  * is its deterministic part alone (advection, drifts, energy change). */
 #define ORACLE_FLAG_NO_NOISE 0x40000000u
+/* gs_params.flags bit read by the oracle only, for timing the reference's
+ * CPU path as it runs: normals from a per-thread std::mt19937 through
+ * std::normal_distribution<float> (OneDimensionBpCpuModel.cpp:60-62), and
+ * threads spawned and joined per iteration of 101 x 250 particles each
+ * (:27-41).  Results then depend on the thread count; no parity test sets it. */
+#define ORACLE_FLAG_FAITHFUL 0x20000000u
 
 #ifdef __cplusplus
 extern "C" {

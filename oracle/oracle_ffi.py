@@ -9,6 +9,11 @@ import subprocess
 
 from geliosphere_amd.abi import GsParams, GsSpectrum, GsExitRecord
 
+# gs_params.flags bits read by the oracle only (oracle.h)
+FLAG_PERTURB = 0x80000000
+FLAG_NO_NOISE = 0x40000000
+FLAG_FAITHFUL = 0x20000000
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib = None

@@ -373,3 +373,33 @@ def test_flop_count_per_step(oracle):
     assert got[abi.MODEL_1D_FP]["flops_per_step"] == pytest.approx(42.0, abs=0.01)
     assert got[abi.MODEL_2D_SOLARPROP]["flops_per_step"] == pytest.approx(164.0, abs=0.01)
     assert 255.0 < got[abi.MODEL_2D_GELIOSPHERE]["flops_per_step"] < 262.0
+
+
+def test_faithful_timing_mode_samples_the_same_physics(oracle):
+    """The reference-faithful timing mode (std::mt19937 +
+    std::normal_distribution<float>, threads per iteration of 101 x 250
+    particles) integrates the same model with another noise source: the
+    same particles are injected (per-bin counts equal), the per-bin mean
+    weights agree with the Philox mode within 4 combined standard errors,
+    the noise is really drawn (spectra differ), and a rerun with the same
+    seed and thread count repeats it exactly."""
+    from oracle import oracle_ffi
+    p = abi.make_params(abi.MODEL_1D_BP, N=4000, seed=41, r_inj=60.0)
+    q = abi.clone_params(p, flags=oracle_ffi.FLAG_FAITHFUL)
+    rc1, a, _, _ = oracle.run(p, n_threads=2)
+    rc2, b, _, _ = oracle.run(q, n_threads=2)
+    rc3, c, _, _ = oracle.run(q, n_threads=2)
+    assert rc1 == rc2 == rc3 == 0 and a.n_particles == b.n_particles == 4000
+    assert bytes(b) == bytes(c)
+    assert bytes(a) != bytes(b)
+    zs = []
+    for k in range(30):
+        assert a.count[k] == b.count[k]
+        n = a.count[k]
+        if n < 30:
+            continue
+        ma, mb = a.sum_w[0][k] / n, b.sum_w[0][k] / n
+        va = max(a.sum_w2[0][k] / n - ma * ma, 0.0) / n
+        vb = max(b.sum_w2[0][k] / n - mb * mb, 0.0) / n
+        zs.append((ma - mb) / math.sqrt(va + vb + 1e-300))
+    assert len(zs) >= 25 and max(abs(z) for z in zs) < 4.0, zs
