@@ -1,7 +1,7 @@
 """The trained surrogate against Monte Carlo.
 
 geliosphere_amd/data holds spectra made by the batched engine (training: 1D
-B-p 4000 at 1e6 particles, SolarProp-like 4000 and Geliosphere 2000 at 2e5;
+B-p 4000 at 1e6 particles, SolarProp-like 6000 and Geliosphere 2000 at 2e5;
 evaluation: 60 held-out per model at 10x those counts, a set that took no
 part in choosing the training configuration) and the ensemble trained on
 them.  For every model it serves (1D B-p, SolarProp-like,
