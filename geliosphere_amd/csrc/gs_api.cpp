@@ -1091,11 +1091,13 @@ gs_status gs_autotune_save(const gs_ctx *ctx, const char *path) {
             char arch[256], kind[16];
             int cus, model;
             if (std::sscanf(buf, "%255s %d %15s %d", arch, &cus, kind, &model) == 4 && ctx->arch == arch &&
-                cus == ctx->n_cu && model >= 0 && model < 4) {
+                cus == ctx->n_cu && model >= 0 && model < 4 &&
+                (std::strcmp(kind, "batch") == 0 || std::strcmp(kind, "plain") == 0)) {
                 const int k = std::strcmp(kind, "batch") == 0 ? 1 : 0;
                 if (ctx->tuned_rate[k][model] > 0) continue;   // replaced below
             }
             lines.emplace_back(buf);
+            if (lines.back().empty() || lines.back().back() != '\n') lines.back() += '\n';
         }
         std::fclose(in);
     }
@@ -1130,7 +1132,9 @@ gs_status gs_autotune_load(gs_ctx *ctx, const char *path, uint32_t *n_applied) {
                         &rate) != 8)
             continue;
         if (ctx->arch != arch || cus != ctx->n_cu || model < 0 || model > 3) continue;
-        const int k = std::strcmp(kind, "batch") == 0 ? 1 : 0;
+        const bool is_batch = std::strcmp(kind, "batch") == 0;
+        if (!is_batch && std::strcmp(kind, "plain") != 0) continue;   // not a kernel this library has
+        const int k = is_batch ? 1 : 0;
         const gs_launch_shape sh{blk, bpcu, chunk};
         if (gs_set_launch_shape(ctx, model, k, &sh) != GS_OK) continue;
         ctx->tuned_rate[k][model] = rate;

@@ -270,3 +270,40 @@ def test_autotuner_searches_shapes_and_caches_them(engine, tmp_path):
     finally:
         engine.set_launch_shape(abi.MODEL_1D_BP, abi.KERNEL_PLAIN)
         engine.set_launch_shape(abi.MODEL_1D_BP, abi.KERNEL_BATCH)
+
+
+def test_autotune_cache_file_rules(engine, tmp_path):
+    """The cache applies only well-formed lines of this device (architecture
+    name and CU count) for the plain and batch kernels, and saving keeps the
+    lines of other devices, including a last line without a newline."""
+    from geliosphere_amd.engine import Engine
+    path = tmp_path / "cache.txt"
+    with Engine(0) as e:
+        arch_line = None
+        # learn this device's key from the save of one tuned shape
+        p = abi.make_params(abi.MODEL_1D_BP, N=50_000, seed=3)
+        e.autotune_shape([p], abi.KERNEL_PLAIN, 1)
+        e.autotune_save(str(path))
+        for l in open(path):
+            if not l.startswith("#"):
+                arch_line = l.split()
+        arch, cus = arch_line[0], arch_line[1]
+    text = ("# header\n"
+            "%s %s plain 3 512 1 128 1e10\n" % (arch, cus) +         # applies
+            "%s %s weird 2 256 1 64 1e10\n" % (arch, cus) +          # unknown kernel: skipped
+            "%s %s batch 2 256\n" % (arch, cus) +                     # malformed: skipped
+            "gfx000 %s batch 1 128 2 16 1e10\n" % cus +               # another device: skipped
+            "%s 1 batch 1 128 2 16 1e10" % arch)                      # another CU count, no newline
+    path.write_text(text)
+    with Engine(0) as e:
+        assert e.autotune_load(str(path)) == 1
+        assert e.launch_shape(abi.MODEL_2D_GELIOSPHERE, abi.KERNEL_PLAIN) == (512, 1, 128)
+        e.autotune_shape([p], abi.KERNEL_PLAIN, 1)
+        e.autotune_save(str(path))
+    lines = open(path).read().splitlines()
+    assert "gfx000 %s batch 1 128 2 16 1e10" % cus in lines
+    assert "%s 1 batch 1 128 2 16 1e10" % arch in lines
+    mine = [l.split() for l in lines if l.startswith(arch + " " + cus + " ")]
+    assert sorted((l[2], l[3]) for l in mine if len(l) == 8 and l[2] in ("plain", "batch")) == \
+        [("plain", "1"), ("plain", "3")]
+    assert "%s %s batch 2 256" % (arch, cus) in lines      # lines it cannot read are kept as they are
