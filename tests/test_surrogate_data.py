@@ -24,9 +24,15 @@ def test_dataset_inputs_are_the_features_of_its_simulations():
     assert np.array_equal(np.array([surrogate.features(p) for p in tr], dtype=np.float32), d["x"])
     assert np.array_equal(np.array([surrogate.features(p) for p in ev], dtype=np.float32), d["ex"])
     assert np.array_equal(d["m"], np.array([p.model for p in tr])) and np.array_equal(d["em"], np.array([p.model for p in ev]))
-    for m, n_tr, p_tr, n_ev, p_ev, seed in d["sets"].tolist():
-        assert d["n_train"][m] == p_tr and d["n_eval"][m] == p_ev
-        assert all(p.n_particles == p_tr for p in tr if p.model == m)
+    # per model, the largest particle count of its parts
+    for m in set(d["m"].tolist()) | set(d["em"].tolist()):
+        rows = [r for r in d["sets"].tolist() if r[0] == m]
+        assert d["n_train"][m] == max(r[2] for r in rows if r[1] > 0 or r[3] > 0)
+        assert d["n_eval"][m] == max(r[4] for r in rows)
+    # each part's simulations carry its particle counts
+    for r in d["sets"].tolist():
+        part_tr, part_ev = sd.redraw([r])
+        assert all(p.n_particles == r[2] for p in part_tr) and all(p.n_particles == r[4] for p in part_ev)
 
 
 def test_weights_match_the_input_width():
