@@ -1,9 +1,10 @@
 """The trained surrogate against Monte Carlo.
 
 geliosphere_amd/data holds spectra made by the batched engine (training: 1D
-B-p 4000 at 1e6 particles, SolarProp-like 6000 and Geliosphere 2000 at 2e5;
-evaluation: 60 held-out per model at 10x those counts, a set that took no
-part in choosing the training configuration) and the ensemble trained on
+B-p 4000 at 1e6 particles, SolarProp-like 6000 at 2e5 and 1000 at 1e6,
+Geliosphere 2000 at 2e5;
+evaluation: 60 held-out per model, B-p at 1e7 and the 2D models at 2e6
+particles, a set that took no part in choosing the training configuration) and the ensemble trained on
 them.  For every model it serves (1D B-p, SolarProp-like,
 Geliosphere) the ensemble must reproduce the held-out Monte Carlo spectra to
 2 % (median over bins in 0.1-10 GeV), both over all queries and over the
