@@ -1145,6 +1145,52 @@ gs_status gs_autotune_load(gs_ctx *ctx, const char *path, uint32_t *n_applied) {
     return GS_OK;
 }
 
+gs_status gs_spectrum_merge(const gs_spectrum *parts, uint32_t n, gs_spectrum *out) {
+    if (!parts || !out || n == 0) return GS_ERR_INVALID_ARG;
+    const uint32_t nb = parts[0].n_bins, nw = parts[0].n_weights;
+    if (nb > GS_MAX_BINS || nw > GS_MAX_WEIGHTS) return GS_ERR_INVALID_ARG;
+    for (uint32_t j = 1; j < n; ++j)
+        if (parts[j].n_bins != nb || parts[j].n_weights != nw) return GS_ERR_INVALID_ARG;
+    gs_spectrum *m = new gs_spectrum;   // out may alias a part
+    std::memset(m, 0, sizeof *m);
+    m->n_bins = nb;
+    m->n_weights = nw;
+    for (uint32_t j = 0; j < n; ++j) {
+        const gs_spectrum &p = parts[j];
+        for (uint32_t b = 0; b < nb; ++b) m->count[b] += p.count[b];
+        m->n_particles += p.n_particles;
+        m->n_events += p.n_events;
+        m->n_outside += p.n_outside;
+        m->total_steps += p.total_steps;
+        m->n_nan += p.n_nan;
+        m->n_step_cap += p.n_step_cap;
+        m->n_acc_inexact += p.n_acc_inexact;
+        m->n_acc_overflow += p.n_acc_overflow;
+        for (uint32_t k = 0; k < nw; ++k)
+            for (int s = 0; s < 2; ++s)
+                for (uint32_t b = 0; b < nb; ++b) {
+                    uint32_t *L = m->acc[k][s][b];
+                    const uint32_t *A = p.acc[k][s][b];
+                    uint64_t carry = 0;
+                    for (int i = 0; i < GS_ACC_LIMBS; ++i) {
+                        const uint64_t v = (uint64_t)L[i] + A[i] + carry;
+                        L[i] = (uint32_t)v;
+                        carry = v >> 32;
+                    }
+                    if (carry) ++m->n_acc_overflow;
+                }
+    }
+    for (uint32_t k = 0; k < nw; ++k)
+        for (uint32_t b = 0; b < nb; ++b) {
+            m->sum_w[k][b] = limbs_to_double(m->acc[k][0][b]);
+            m->sum_w2[k][b] = limbs_to_double(m->acc[k][1][b]);
+        }
+    const bool overflow = m->n_acc_overflow != 0;
+    *out = *m;
+    delete m;
+    return overflow ? GS_ERR_OVERFLOW : GS_OK;
+}
+
 gs_status gs_last_run_stats(const gs_ctx *ctx, gs_run_stats *st) {
     if (!ctx || !st) return GS_ERR_INVALID_ARG;
     *st = ctx->stats;

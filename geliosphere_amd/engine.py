@@ -18,7 +18,7 @@ EXPORTS = ("gs_abi_version", "gs_create", "gs_destroy", "gs_last_error", "gs_def
            "gs_set_plain_limits",
            "gs_set_launch", "gs_set_launch_shape", "gs_get_launch_shape", "gs_set_order", "gs_autotune",
            "gs_autotune_shape", "gs_autotune_save", "gs_autotune_load", "gs_last_run_stats", "gs_debug_math",
-           "gs_debug_normals", "gs_host_math")
+           "gs_debug_normals", "gs_host_math", "gs_spectrum_merge")
 
 ORDER_ID, ORDER_ENERGY = 0, 1
 
@@ -94,6 +94,8 @@ def lib():
         L.gs_debug_normals.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, dp]
         L.gs_debug_normals.restype = ctypes.c_int
         L.gs_host_math.argtypes = [ctypes.c_int32, dp, dp, ctypes.c_uint64]
+        L.gs_spectrum_merge.argtypes = [P(abi.GsSpectrum), ctypes.c_uint32, P(abi.GsSpectrum)]
+        L.gs_spectrum_merge.restype = ctypes.c_int
         L.gs_host_math.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -110,6 +112,16 @@ def host_math(fn, xs):
     if rc:
         raise GsError(rc, "gs_host_math")
     return list(y)
+
+
+def merge_spectra(parts):
+    """gs_spectrum_merge: the spectrum of a run made of these id-range shards
+    (host only).  Returns (status, spectrum)."""
+    n = len(parts)
+    arr = (abi.GsSpectrum * n)(*parts)
+    out = abi.GsSpectrum()
+    rc = lib().gs_spectrum_merge(arr, n, ctypes.byref(out))
+    return rc, out
 
 
 def validate(p):

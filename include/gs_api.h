@@ -306,6 +306,19 @@ gs_status gs_autotune_load(gs_ctx *ctx, const char *path, uint32_t *n_applied);
 /* Statistics of the last run on ctx. */
 gs_status gs_last_run_stats(const gs_ctx *ctx, gs_run_stats *st);
 
+/* The spectrum of a run made of n parts (id-range shards of one simulation,
+ * e.g. one per GPU or per call, first_particle/n_particles apart): counts,
+ * statistics and exact sums added limb by limb, sum_w/sum_w2 converted with
+ * correct rounding.  Equal to the spectrum of the whole run bit for bit,
+ * whatever the order of the parts.  Host only: needs no context or device.
+ * Parts must agree in n_bins and n_weights (GS_ERR_INVALID_ARG otherwise);
+ * a carry out of the top limb counts in n_acc_overflow and returns
+ * GS_ERR_OVERFLOW.  out may alias parts[0].  Replaces the reference's
+ * single-process accumulation of log.dat lines by the Results pass
+ * (Algorithm/src/OneDimensionBpResults.cpp:52-87), for runs split across
+ * devices. */
+gs_status gs_spectrum_merge(const gs_spectrum *parts, uint32_t n, gs_spectrum *out);
+
 /* ------------------------------------------------------------------------
  * Diagnostics.  No counterpart in the reference: they expose the engine's
  * FP64 math layer and normal stream so that tests can pin them (to the ulp

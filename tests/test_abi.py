@@ -80,3 +80,45 @@ def test_create_without_device_fails_cleanly():
     ctx = ctypes.c_void_p()
     assert engine.lib().gs_create(ctypes.byref(ctx), 0) == abi.GS_ERR_NO_DEVICE
     assert not ctx.value
+
+
+def test_spectrum_merge_of_shards_equals_whole_run():
+    """gs_spectrum_merge (host only) adds id-range shards of one run into the
+    whole run's spectrum bit for bit, in any order.  The shards come from
+    the oracle, whose spectra have the same exact-sum layout."""
+    from oracle import oracle_ffi
+    p = abi.make_params(abi.MODEL_1D_BP, N=3000, seed=17, r_inj=40.0,
+                        weights=(abi.W_POWERLAW, abi.W_YAMADA, abi.W_WEBER, abi.W_USOSKIN))
+    rc, whole, _, _ = oracle_ffi.run(p, n_threads=4)
+    assert rc == 0
+    cuts = [0, 700, 1900, 3000]
+    parts = []
+    for a, b in zip(cuts, cuts[1:]):
+        rc, s, _, _ = oracle_ffi.run(abi.clone_params(p, first_particle=a, n_particles=b - a), n_threads=4)
+        assert rc == 0
+        parts.append(s)
+    for order in ([0, 1, 2], [2, 0, 1]):
+        rc, m = engine.merge_spectra([parts[i] for i in order])
+        assert rc == abi.GS_OK
+        assert bytes(m) == bytes(whole)
+    # a single part is returned as it is; mismatched layouts are refused
+    rc, m = engine.merge_spectra([whole])
+    assert rc == abi.GS_OK and bytes(m) == bytes(whole)
+    other = abi.GsSpectrum()
+    ctypes.memmove(ctypes.byref(other), ctypes.byref(parts[0]), ctypes.sizeof(other))
+    other.n_bins = 12
+    rc, _ = engine.merge_spectra([parts[0], other])
+    assert rc == abi.GS_ERR_INVALID_ARG
+
+
+def test_spectrum_merge_reports_overflow():
+    """A carry out of the top limb is counted and reported, not dropped."""
+    a = abi.GsSpectrum()
+    a.n_bins, a.n_weights = 1, 1
+    for i in range(abi.GS_ACC_LIMBS):
+        a.acc[0][0][0][i] = 0xFFFFFFFF
+    b = abi.GsSpectrum()
+    b.n_bins, b.n_weights = 1, 1
+    b.acc[0][0][0][0] = 1
+    rc, m = engine.merge_spectra([a, b])
+    assert rc == abi.GS_ERR_OVERFLOW and m.n_acc_overflow == 1

@@ -219,7 +219,8 @@ def test_result_independent_of_launch_geometry(engine, model):
 
 def test_shards_add_exactly(engine):
     """Particles are keyed by id, and sums are exact integers, so two shards
-    of the id range add up to the whole run bit for bit."""
+    of the id range add up to the whole run bit for bit (gs_spectrum_merge,
+    the reduction a run split across GPUs would use)."""
     n = 100_000
     p = abi.make_params(abi.MODEL_1D_BP, N=n, seed=8, weights=ALL_WEIGHTS)
     whole, _, _ = engine.run_plain(p)
@@ -231,6 +232,10 @@ def test_shards_add_exactly(engine):
             for s in (0, 1):
                 assert abi.acc_int(a, k, s, bin_) + abi.acc_int(b, k, s, bin_) == abi.acc_int(whole, k, s, bin_)
     assert a.total_steps + b.total_steps == whole.total_steps
+    # the library's reduction of the shards is the whole run, bit for bit
+    from geliosphere_amd.engine import merge_spectra
+    rc, m = merge_spectra([b, a])
+    assert rc == abi.GS_OK and bytes(m) == bytes(whole)
 
 
 def test_exact_sums_convert_correctly(engine):
