@@ -309,8 +309,9 @@ def main():
         calibrate_copy(a.calibrate_copy)
         return
     if a.gpus != 1:
-        print("bench.py runs on one GPU only: the engine has no multi-GPU path "
-              "(a run would shard by particle id range, one range per GPU)", file=sys.stderr)
+        print("bench.py measures one GPU only: a run on several GPUs would shard by particle id "
+              "range, one range per GPU, and add the parts with gs_spectrum_merge; no multi-GPU "
+              "launcher is part of this build", file=sys.stderr)
         sys.exit(2)
     from geliosphere_amd import abi
     from geliosphere_amd.engine import Engine
