@@ -768,7 +768,7 @@ using namespace gs;
 #define GS_PLAIN_WAVES_SP 1
 #endif
 #ifndef GS_BATCH_WAVES_BP
-#define GS_BATCH_WAVES_BP 5   // 96 VGPRs (17 spilled); 4 waves (121, none) measured 0.8 % faster, within the tuning noise
+#define GS_BATCH_WAVES_BP 5   // 96 VGPRs (17 spilled); 4 waves (121, none) ran 0.8 % faster on 32 B-p simulations at 30 % of the batch benchmark's counts, not switched: the tuned batch shapes were measured at 5
 #endif
 constexpr int kBatchWaves1D = 1, kBatchWavesBp = GS_BATCH_WAVES_BP, kBatchWavesSp = 2, kBatchWavesGe = GS_BATCH_WAVES_GE;
 constexpr int kPlainWavesBp = GS_PLAIN_WAVES_BP;
